@@ -1,0 +1,133 @@
+/*
+ * wotk.h -- C ABI of the MI355X unbalanced entropic OT library (libwotk.so).
+ *
+ * Drop-in boundary for wot's optimal-transport hot path.  The reference is
+ * pure Python; its "FFI" is the solver callable that OTModel invokes as
+ * ``compute_transport_matrix(solver=self.solver, **config)``
+ * (wot/ot/ot_model.py:318 -> wot/ot/optimal_transport.py:30).  Each entry point
+ * below replaces one reference call site; the Python package wot_amd binds
+ * them with ctypes (wot_amd/_native.py) behind the reference's own signatures.
+ *
+ * Conventions
+ *   - Every pointer argument is DEVICE memory unless stated otherwise.
+ *   - Matrices are row-major with an explicit leading dimension (elements).
+ *     Single-precision cost matrices must have ld % 4 == 0 and 16-byte
+ *     aligned base pointers (the sweep kernels read float4).
+ *   - Every call takes a hipStream_t (passed as void*) and enqueues on it.
+ *     Solver calls synchronise that stream where the algorithm needs a value
+ *     on the host (duality-gap checks) and before returning.
+ *   - The library never allocates device memory: scratch space comes in a
+ *     caller-owned workspace whose size the matching *_workspace_size()
+ *     returns.
+ *   - Return value: WOTK_OK (0) or an error code; wotk_last_error() gives the
+ *     message of the last failure in the calling thread.
+ *   - Calls that share a workspace or outputs must not overlap.
+ */
+#ifndef WOTK_H
+#define WOTK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WOTK_OK 0
+#define WOTK_EINVAL 1     /* bad argument (ValueError on the Python side) */
+#define WOTK_EHIP 2       /* HIP runtime failure                          */
+#define WOTK_ENAN 3       /* NaN duality gap (optimal_transport.py:162-163) */
+#define WOTK_EWORKSPACE 4 /* workspace too small                          */
+
+/* cost modes for wotk_cost_sqeuclid */
+#define WOTK_COST_F64 0       /* FP64 direct difference sum_k (x_k - y_k)^2     */
+#define WOTK_COST_BF16X3 1    /* split-BF16 3-term MFMA cross term, FP32 accum  */
+
+typedef struct wotk_stats {
+    int64_t iterations;       /* Sinkhorn (a, b) iterations performed            */
+    int64_t stabilisations;   /* tau-triggered absorptions                      */
+    int64_t gap_checks;       /* duality-gap / dual-evolution checks            */
+    double gap;               /* last duality gap (or dual-evolution measure)  */
+    double epsilon_final;     /* epsilon the returned plan is built with        */
+    int32_t max_iter_reached; /* 1: early un-normalised return (warning)        */
+    int32_t nonfinite;        /* 1: a potential became non-finite               */
+    double primal;            /* last primal objective (duality-gap solver)     */
+    double dual;              /* last dual objective                            */
+    int64_t sweeps;           /* row log-sum-exp sweeps launched (2 per iteration) */
+    double sweep_ms;          /* device time of the fixed-iteration sweep loop (hipEvents) */
+} wotk_stats;
+
+const char *wotk_last_error(void);
+int wotk_abi_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Cost matrix.  Replaces OTModel.compute_default_cost_matrix
+ * (wot/ot/ot_model.py:242-253): C = cdist(X, Y, 'sqeuclidean') / median(C).
+ * X: (I, d) and Y: (J, d) float64 row-major (ld = d).  Writes the normalised
+ * cost in FP32 twice, C (I x J, ldc) and its transpose CT (J x I, ldct), and the
+ * FP64 median to *median_out (device scalar).  mode = WOTK_COST_F64 computes
+ * the FP64 direct difference (scipy's formula); WOTK_COST_BF16X3 computes
+ * ||x||^2 + ||y||^2 - 2 x.y with the cross term as a 3-term split-BF16 MFMA
+ * GEMM accumulated in FP32.  The median is exact (radix select over the FP64
+ * bit patterns; even count -> mean of the two middle values, numpy.median).
+ * ------------------------------------------------------------------------- */
+size_t wotk_cost_workspace_size(int64_t I, int64_t J, int64_t d, int32_t mode);
+int wotk_cost_sqeuclid(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
+                       int32_t mode, float *C, int64_t ldc, float *CT, int64_t ldct,
+                       double *median_out, void *workspace, size_t workspace_bytes, void *stream);
+
+/* Un-normalised FP64 squared distances (I x J, ldc64), no median: the raw
+ * pairwise_distances(..., 'sqeuclidean') of ot_model.py:249-251. */
+int wotk_sqeuclid_f64(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
+                      double *C64, int64_t ldc64, void *stream);
+
+/* Exact median of n FP64 values >= 0 (np.median, ot_model.py:252). */
+size_t wotk_median_workspace_size(int64_t n);
+int wotk_median_f64(const double *values, int64_t n, double *median_out, void *workspace,
+                    size_t workspace_bytes, void *stream);
+
+/* A caller-supplied cost (OTModel.compute_transport_map(..., cost_matrix=C),
+ * ot_model.py:203-240, used as-is by the solvers) converted to the FP32 pair
+ * (C, CT).  C64 is (I x J) with leading dimension ld64, scale multiplies every
+ * entry (1.0 for an as-is cost). */
+int wotk_cost_import(const double *C64, int64_t I, int64_t J, int64_t ld64, double scale,
+                     float *C, int64_t ldc, float *CT, int64_t ldct, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Solvers.  C (I x J, ldc) and CT (J x I, ldct) hold the same FP32 cost.
+ * G (I) float64 growth.  plan (I x J, ldp) float64 output, may be NULL;
+ * row_sums (I) float64 output of the returned plan, may be NULL (the growth
+ * loop's next G, optimal_transport.py:27).  f_out (I) / g_out (J) receive the
+ * final dual potentials u + eps log a, v + eps log b, may be NULL.
+ * ------------------------------------------------------------------------- */
+size_t wotk_solver_workspace_size(int64_t I, int64_t J);
+
+/* transport_stablev2 (wot/ot/optimal_transport.py:167-236), log-domain. */
+int wotk_transport_stablev2(const float *C, int64_t ldc, const float *CT, int64_t ldct, int64_t I,
+                            int64_t J, const double *G, double lambda1, double lambda2,
+                            double epsilon, int64_t scaling_iter, double tau, double epsilon0,
+                            int64_t extra_iter, int64_t inner_iter_max, double *plan, int64_t ldp,
+                            double *row_sums, double *f_out, double *g_out, void *workspace,
+                            size_t workspace_bytes, wotk_stats *stats, void *stream);
+
+/* optimal_transport_duality_gap (wot/ot/optimal_transport.py:67-164). */
+int wotk_transport_duality_gap(const float *C, int64_t ldc, const float *CT, int64_t ldct,
+                               int64_t I, int64_t J, const double *G, double lambda1,
+                               double lambda2, double epsilon, int64_t batch_size,
+                               double tolerance, double tau, double epsilon0, int64_t max_iter,
+                               double *plan, int64_t ldp, double *row_sums, double *f_out,
+                               double *g_out, void *workspace, size_t workspace_bytes,
+                               wotk_stats *stats, void *stream);
+
+/* Primal / dual objectives of optimal_transport.py:45-62 at the plan
+ * R = exp((f_i + g_j - C_ij)/eps) (un-normalised), FP64 reductions.
+ * out[0] = primal, out[1] = dual (device or host pointer to 2 doubles: host). */
+int wotk_primal_dual(const float *C, int64_t ldc, const float *CT, int64_t ldct, int64_t I,
+                     int64_t J, const double *G, const double *f, const double *g,
+                     double lambda1, double lambda2, double epsilon, double *out_host,
+                     void *workspace, size_t workspace_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WOTK_H */

@@ -1,0 +1,267 @@
+"""FP64 CPU oracle for wot's unbalanced entropic OT path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product (``wot_amd``) imports this
+module; it is used by ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` as the checker / CPU baseline.
+
+It restates, in plain numpy float64, the algorithm of the reference solver so
+that the GPU path can be compared against it on identical seeded inputs.  The
+statement order follows the reference file by file so that a fixed iteration
+budget gives the same trajectory (stabilisation points, epsilon schedule,
+check cadence):
+
+* ``compute_transport_matrix``      wot/ot/optimal_transport.py:10-33
+* ``fdiv`` / ``fdivstar``           wot/ot/optimal_transport.py:37-42
+* ``primal`` / ``dual``             wot/ot/optimal_transport.py:45-62
+* ``optimal_transport_duality_gap`` wot/ot/optimal_transport.py:67-164
+* ``transport_stablev2``            wot/ot/optimal_transport.py:167-236
+* ``default_cost_matrix``           wot/ot/ot_model.py:242-253 (sklearn
+  ``pairwise_distances(metric='sqeuclidean')`` resolves to scipy ``cdist``)
+* ``glue`` / ``chain``              wot/tmap/util.py:74-94, wot/tmap/chaining.py:4-40
+
+Pinning (how each function is checked independently of this file):
+golden vectors written from the reference file itself (tests/golden/), the
+library routines scipy.cdist / numpy.median, closed forms (I=J=1 fixed point,
+balanced-limit Sinkhorn), weak duality, and brute-force convex minimisation
+on tiny problems.  See tests/test_oracle.py.
+"""
+import logging
+import warnings
+
+import numpy as np
+
+logger = logging.getLogger('wot')
+
+
+# --------------------------------------------------------------------------
+# growth outer loop                      (wot/ot/optimal_transport.py:10-33)
+# --------------------------------------------------------------------------
+def compute_transport_matrix(solver, **params):
+    """Growth-learning outer loop.  Iteration 0 solves with the given G,
+    iteration k>0 with the previous plan's row sums.  ``params['G']`` is
+    overwritten in the caller's dict, as in the reference (:28)."""
+    growth = params['G']
+    n_outer = params['growth_iters']
+    learned = []
+    tmap = None
+    for it in range(n_outer):
+        rows = growth if it == 0 else tmap.sum(axis=1)
+        params['G'] = rows
+        learned.append(rows)
+        tmap = solver(**params)
+    return tmap, learned
+
+
+# --------------------------------------------------------------------------
+# Chizat primal / dual                   (wot/ot/optimal_transport.py:37-62)
+# --------------------------------------------------------------------------
+def fdiv(l, x, p, dx):
+    """KL-type marginal penalty  l * sum dx (x log(x/p) - x + p)   (:37-38)."""
+    return l * np.sum(dx * (x * np.log(x / p) - x + p))
+
+
+def fdivstar(l, u, p, dx):
+    """Convex conjugate  l * sum (p dx)(exp(u/l) - 1)              (:41-42)."""
+    return l * np.sum((p * dx) * (np.exp(u / l) - 1.0))
+
+
+def primal(C, K, R, dx, dy, p, q, a, b, epsilon, lambda1, lambda2):
+    """Regularised unbalanced primal objective                      (:45-53)."""
+    n_i, n_j = len(p), len(q)
+    with np.errstate(divide='ignore'):
+        logr = np.nan_to_num(np.log(R))
+        ent = epsilon * np.sum(R * logr - R + K)
+    return (fdiv(lambda1, R.dot(dy), p, dx) + fdiv(lambda2, R.T.dot(dx), q, dy)
+            + (ent + np.sum(R * C)) / (n_i * n_j))
+
+
+def dual(C, K, R, dx, dy, p, q, a, b, epsilon, lambda1, lambda2):
+    """Dual objective in terms of the true scalings a, b            (:56-62)."""
+    n_i, n_j = len(p), len(q)
+    return (-fdivstar(lambda1, -epsilon * np.log(a), p, dx)
+            - fdivstar(lambda2, -epsilon * np.log(b), q, dy)
+            - epsilon * np.sum(R - K) / (n_i * n_j))
+
+
+def _stab_kernel(u, v, C, eps):
+    return np.exp((u[:, None] - C + v[None, :]) / eps)
+
+
+# --------------------------------------------------------------------------
+# duality-gap solver (OTModel default)  (wot/ot/optimal_transport.py:67-164)
+# --------------------------------------------------------------------------
+def optimal_transport_duality_gap(C, G, lambda1, lambda2, epsilon, batch_size, tolerance, tau,
+                                  epsilon0, max_iter, stats=None, **ignored):
+    """Five epsilon scalings eps0*eps^(e/5), e=0..5; batches of 5 (or
+    ``batch_size`` on the last scaling) between checks; dual-variable evolution
+    check (threshold 1e-6) on early scalings, relative duality gap
+    (threshold ``tolerance``) on the last.  Returns R/J; on ``max_iter`` returns
+    the un-normalised plan with a warning; NaN gap raises RuntimeError."""
+    C = np.asarray(C, dtype=np.float64)
+    n_scal = 5
+    factor = np.exp(-np.log(epsilon) / n_scal)
+    n_i, n_j = C.shape
+    dx = np.ones(n_i) / n_i
+    dy = np.ones(n_j) / n_j
+    p = G
+    q = np.ones(n_j) * np.average(G)
+    u = np.zeros(n_i)
+    v = np.zeros(n_j)
+    a = np.ones(n_i)
+    b = np.ones(n_j)
+    eps_i = epsilon0 * factor
+    n_iter = 0
+    n_stab = 0
+    gap = np.inf
+    R = None
+    for e in range(n_scal + 1):
+        gap = np.inf
+        u = u + eps_i * np.log(a)
+        v = v + eps_i * np.log(b)
+        eps_i = eps_i / factor
+        Kt = np.exp(-C / eps_i)
+        al1 = lambda1 / (lambda1 + eps_i)
+        al2 = lambda2 / (lambda2 + eps_i)
+        K = _stab_kernel(u, v, C, eps_i)
+        a = np.ones(n_i)
+        b = np.ones(n_j)
+        old_a, old_b = a, b
+        thr = tolerance if e == n_scal else 1e-6
+        while gap > thr:
+            n_inner = batch_size if e == n_scal else 5
+            for _ in range(n_inner):
+                n_iter += 1
+                old_a, old_b = a, b
+                a = (p / K.dot(b * dy)) ** al1 * np.exp(-u / (lambda1 + eps_i))
+                b = (q / K.T.dot(a * dx)) ** al2 * np.exp(-v / (lambda2 + eps_i))
+                if max(np.max(np.abs(a)), np.max(np.abs(b))) > tau:
+                    u = u + eps_i * np.log(a)
+                    v = v + eps_i * np.log(b)
+                    K = _stab_kernel(u, v, C, eps_i)
+                    a = np.ones(n_i)
+                    b = np.ones(n_j)
+                    n_stab += 1
+                if n_iter >= max_iter:
+                    logger.warning("Reached max_iter with duality gap still above threshold. Returning")
+                    if stats is not None:
+                        stats.update(iterations=n_iter, stabilisations=n_stab, gap=gap,
+                                     epsilon_final=eps_i, max_iter_reached=True)
+                    return (K.T * a).T * b
+            a_true = a * np.exp(u / eps_i)
+            b_true = b * np.exp(v / eps_i)
+            if e == n_scal:
+                R = (K.T * a).T * b
+                pri = primal(C, Kt, R, dx, dy, p, q, a_true, b_true, eps_i, lambda1, lambda2)
+                dua = dual(C, Kt, R, dx, dy, p, q, a_true, b_true, eps_i, lambda1, lambda2)
+                gap = (pri - dua) / abs(pri)
+            else:
+                gap = max(
+                    np.linalg.norm(a_true - old_a * np.exp(u / eps_i)) / (1 + np.linalg.norm(a_true)),
+                    np.linalg.norm(b_true - old_b * np.exp(v / eps_i)) / (1 + np.linalg.norm(b_true)))
+    if np.isnan(gap):
+        raise RuntimeError("Overflow encountered in duality gap computation, please report this incident")
+    if stats is not None:
+        stats.update(iterations=n_iter, stabilisations=n_stab, gap=gap, epsilon_final=eps_i,
+                     max_iter_reached=False)
+    return R / C.shape[1]
+
+
+# --------------------------------------------------------------------------
+# fixed-iteration solver                (wot/ot/optimal_transport.py:167-236)
+# --------------------------------------------------------------------------
+def transport_stablev2(C, lambda1, lambda2, epsilon, scaling_iter, G, tau, epsilon0, extra_iter,
+                       inner_iter_max, stats=None, **ignored):
+    """``scaling_iter`` iterations with the epsilon schedule
+    eps(n) = (eps0-eps) e^-n + eps changed every ``inner_iter_max`` iterations
+    and tau-triggered absorption, then ``extra_iter`` plain iterations.
+    Returns R/J."""
+    warm = tau is not None
+    eps_final = epsilon
+
+    def reg(n):
+        return (epsilon0 - eps_final) * np.exp(-n) + eps_final
+
+    eps_i = epsilon0 if warm else epsilon
+    n_i, n_j = C.shape
+    dx = np.ones(n_i) / n_i
+    dy = np.ones(n_j) / n_j
+    p = G
+    q = np.ones(n_j) * np.average(G)
+    u = np.zeros(len(p))
+    v = np.zeros(len(q))
+    b = np.ones(len(q))
+    K = np.exp(-C / eps_i)
+    al1 = lambda1 / (lambda1 + eps_i)
+    al2 = lambda2 / (lambda2 + eps_i)
+    n_eps = 0
+    since = 0
+    n_stab = 0
+    a = None
+    with warnings.catch_warnings():
+        warnings.simplefilter('ignore', RuntimeWarning)
+        for _ in range(scaling_iter):
+            a = (p / K.dot(b * dy)) ** al1 * np.exp(-u / (lambda1 + eps_i))
+            b = (q / K.T.dot(a * dx)) ** al2 * np.exp(-v / (lambda2 + eps_i))
+            since += 1
+            if max(np.max(np.abs(a)), np.max(np.abs(b))) > tau:
+                u = u + eps_i * np.log(a)
+                v = v + eps_i * np.log(b)
+                K = _stab_kernel(u, v, C, eps_i)
+                a = np.ones(len(p))
+                b = np.ones(len(q))
+                n_stab += 1
+            if warm and since == inner_iter_max:
+                n_eps += 1
+                since = 0
+                u = u + eps_i * np.log(a)
+                v = v + eps_i * np.log(b)
+                eps_i = reg(n_eps)
+                al1 = lambda1 / (lambda1 + eps_i)
+                al2 = lambda2 / (lambda2 + eps_i)
+                K = _stab_kernel(u, v, C, eps_i)
+                a = np.ones(len(p))
+                b = np.ones(len(q))
+        for _ in range(extra_iter):
+            a = (p / K.dot(b * dy)) ** al1 * np.exp(-u / (lambda1 + eps_i))
+            b = (q / K.T.dot(a * dx)) ** al2 * np.exp(-v / (lambda2 + eps_i))
+    R = (K.T * a).T * b
+    if stats is not None:
+        stats.update(iterations=scaling_iter + extra_iter, stabilisations=n_stab,
+                     epsilon_final=eps_i, u=u + eps_i * np.log(a), v=v + eps_i * np.log(b))
+    return R / C.shape[1]
+
+
+# --------------------------------------------------------------------------
+# cost                                   (wot/ot/ot_model.py:242-253)
+# --------------------------------------------------------------------------
+def sqeuclidean(x, y):
+    """Direct-difference squared Euclidean distances, as scipy cdist
+    'sqeuclidean' computes them (sum_k (x_k - y_k)^2, float64)."""
+    from scipy.spatial.distance import cdist
+    return cdist(np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64), 'sqeuclidean')
+
+
+def default_cost_matrix(a, b, eigenvals=None):
+    """C = sqeuclid(a, b) / median(C)   (ot_model.py:243-253)."""
+    if eigenvals is not None:
+        a = a.dot(eigenvals)
+        b = b.dot(eigenvals)
+    c = sqeuclidean(a, b)
+    return c / np.median(c)
+
+
+def growth_obs(learned_growth, tmap, delta_days):
+    """obs columns g0..gN of the returned AnnData   (ot_model.py:319-325)."""
+    lg = list(learned_growth) + [tmap.sum(axis=1)]
+    return {'g' + str(i): np.power(g, 1.0 / delta_days) for i, g in enumerate(lg)}
+
+
+# --------------------------------------------------------------------------
+# long-range coupling chain     (wot/tmap/util.py:74-94, tmap/chaining.py:4-40)
+# --------------------------------------------------------------------------
+def chain(maps):
+    """Successive products T0 T1 ... with no normalisation (util.py:93)."""
+    out = np.asarray(maps[0], dtype=np.float64)
+    for m in maps[1:]:
+        out = out.dot(np.asarray(m, dtype=np.float64))
+    return out

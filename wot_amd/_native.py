@@ -1,0 +1,100 @@
+"""ctypes binding of libwotk.so (the C ABI declared in include/wotk.h).
+
+The library is built in-tree by ``python -m wot_amd.build``.  Loading fails
+loudly if it is missing: there is no CPU fallback for the product path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, '_lib', 'libwotk.so')
+
+WOTK_OK = 0
+WOTK_EINVAL = 1
+WOTK_EHIP = 2
+WOTK_ENAN = 3
+WOTK_EWORKSPACE = 4
+
+COST_F64 = 0
+COST_BF16X3 = 1
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [('iterations', ctypes.c_int64),
+                ('stabilisations', ctypes.c_int64),
+                ('gap_checks', ctypes.c_int64),
+                ('gap', ctypes.c_double),
+                ('epsilon_final', ctypes.c_double),
+                ('max_iter_reached', ctypes.c_int32),
+                ('nonfinite', ctypes.c_int32),
+                ('primal', ctypes.c_double),
+                ('dual', ctypes.c_double),
+                ('sweeps', ctypes.c_int64),
+                ('sweep_ms', ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_D = ctypes.c_double
+_SZ = ctypes.c_size_t
+
+# name -> (restype, argtypes); every symbol include/wotk.h declares
+SIGNATURES = {
+    'wotk_last_error': (ctypes.c_char_p, []),
+    'wotk_abi_version': (ctypes.c_int, []),
+    'wotk_cost_workspace_size': (_SZ, [_I64, _I64, _I64, _I32]),
+    'wotk_cost_sqeuclid': (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _I32, _P, _I64, _P, _I64, _P, _P,
+                                          _SZ, _P]),
+    'wotk_sqeuclid_f64': (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _I64, _P]),
+    'wotk_median_workspace_size': (_SZ, [_I64]),
+    'wotk_median_f64': (ctypes.c_int, [_P, _I64, _P, _P, _SZ, _P]),
+    'wotk_cost_import': (ctypes.c_int, [_P, _I64, _I64, _I64, _D, _P, _I64, _P, _I64, _P]),
+    'wotk_solver_workspace_size': (_SZ, [_I64, _I64]),
+    'wotk_transport_stablev2': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64, _D,
+                                               _D, _I64, _I64, _P, _I64, _P, _P, _P, _P, _SZ,
+                                               ctypes.POINTER(Stats), _P]),
+    'wotk_transport_duality_gap': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64,
+                                                  _D, _D, _D, _I64, _P, _I64, _P, _P, _P, _P, _SZ,
+                                                  ctypes.POINTER(Stats), _P]),
+    'wotk_primal_dual': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _P, _P, _D, _D, _D,
+                                        ctypes.POINTER(ctypes.c_double), _P, _SZ, _P]),
+}
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libwotk.so once; raise ImportError if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError('libwotk.so not found at %s; build it with `python -m wot_amd.build`'
+                              % LIB_PATH)
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc):
+    """Map a status code to the reference's exception types."""
+    if rc == WOTK_OK:
+        return
+    msg = lib().wotk_last_error().decode(errors='replace')
+    if rc == WOTK_EINVAL:
+        raise ValueError(msg)
+    if rc == WOTK_ENAN:
+        raise RuntimeError(msg)
+    raise NativeError('libwotk error %d: %s' % (rc, msg))

@@ -1,0 +1,41 @@
+"""Transport-map output (the subset of wot/io/io.py:439-478 the OT driver uses).
+
+'txt' (tab-separated, ``id`` index label) and 'npz' need nothing beyond
+numpy/pandas; 'h5ad' needs the anndata package.
+"""
+import numpy as np
+import pandas as pd
+
+
+def check_file_extension(name, output_format):
+    """Append '.<format>' unless present (io.py:475-478)."""
+    if not str(name).lower().endswith('.' + output_format):
+        name += '.' + output_format
+    return name
+
+
+def write_dataset(ds, path, output_format='txt'):
+    """Write an AnnData-like transport map (io.py:439-452)."""
+    path = check_file_extension(str(path), output_format)
+    if output_format == 'txt':
+        pd.DataFrame(np.asarray(ds.X), index=ds.obs.index, columns=ds.var.index).to_csv(
+            path, index_label='id', sep='\t', doublequote=False)
+    elif output_format == 'h5ad':
+        if not hasattr(ds, 'write'):
+            raise ImportError("writing h5ad needs the anndata package; use output_file_format='npz' or 'txt'")
+        ds.write(path)
+    elif output_format == 'npz':
+        obs = {('obs_' + c): ds.obs[c].values for c in ds.obs.columns}
+        np.savez(path, X=np.asarray(ds.X), obs_index=np.asarray(ds.obs.index.astype(str)),
+                 var_index=np.asarray(ds.var.index.astype(str)), **obs)
+    else:
+        raise ValueError('Unknown file format')
+    return path
+
+
+def read_npz_tmap(path):
+    """Read back a map written with output_format='npz' -> (X, obs DataFrame, var index)."""
+    z = np.load(path, allow_pickle=False)
+    obs = pd.DataFrame({k[4:]: z[k] for k in z.files if k.startswith('obs_') and k != 'obs_index'},
+                       index=pd.Index(z['obs_index']))
+    return z['X'], obs, pd.Index(z['var_index'])
