@@ -65,7 +65,9 @@ int wotk_abi_version(void);
  * (wot/ot/ot_model.py:242-253): C = cdist(X, Y, 'sqeuclidean') / median(C).
  * X: (I, d) and Y: (J, d) float64 row-major (ld = d).  Writes the normalised
  * cost in FP32 twice, C (I x J, ldc) and its transpose CT (J x I, ldct), and the
- * FP64 median to *median_out (device scalar).  mode = WOTK_COST_F64 computes
+ * FP64 median to *median_out (device scalar) and, if median_index_out is not
+ * NULL, the flat indices (i*J + j) of the middle element(s) as in
+ * wotk_median_f64.  mode = WOTK_COST_F64 computes
  * the FP64 direct difference (scipy's formula); WOTK_COST_BF16X3 computes
  * ||x||^2 + ||y||^2 - 2 x.y with the cross term as a 3-term split-BF16 MFMA
  * GEMM accumulated in FP32.  The median is exact (radix select over the FP64
@@ -74,17 +76,22 @@ int wotk_abi_version(void);
 size_t wotk_cost_workspace_size(int64_t I, int64_t J, int64_t d, int32_t mode);
 int wotk_cost_sqeuclid(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
                        int32_t mode, float *C, int64_t ldc, float *CT, int64_t ldct,
-                       double *median_out, void *workspace, size_t workspace_bytes, void *stream);
+                       double *median_out, int64_t *median_index_out, void *workspace,
+                       size_t workspace_bytes, void *stream);
 
 /* Un-normalised FP64 squared distances (I x J, ldc64), no median: the raw
  * pairwise_distances(..., 'sqeuclidean') of ot_model.py:249-251. */
 int wotk_sqeuclid_f64(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
                       double *C64, int64_t ldc64, void *stream);
 
-/* Exact median of n FP64 values >= 0 (np.median, ot_model.py:252). */
+/* Exact median of n FP64 values >= 0 (np.median, ot_model.py:252).
+ * median_index_out (device, 2 entries, may be NULL) receives the flat index of
+ * an element holding the lower and the upper middle value (equal for odd n):
+ * the elements that carry the median's subgradient. */
 size_t wotk_median_workspace_size(int64_t n);
-int wotk_median_f64(const double *values, int64_t n, double *median_out, void *workspace,
-                    size_t workspace_bytes, void *stream);
+int wotk_median_f64(const double *values, int64_t n, double *median_out,
+                    int64_t *median_index_out, void *workspace, size_t workspace_bytes,
+                    void *stream);
 
 /* A caller-supplied cost (OTModel.compute_transport_map(..., cost_matrix=C),
  * ot_model.py:203-240, used as-is by the solvers) converted to the FP32 pair
@@ -126,6 +133,33 @@ int wotk_primal_dual(const float *C, int64_t ldc, const float *CT, int64_t ldct,
                      int64_t J, const double *G, const double *f, const double *g,
                      double lambda1, double lambda2, double epsilon, double *out_host,
                      void *workspace, size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Long-range coupling chain (wot/tmap/util.py:74-94 glue_transport_maps,
+ * np.dot(tmap_0.X, tmap_1.X[idx]); chained by wot/tmap/chaining.py:4-40):
+ * C (M x N, ldc) = A (M x K, lda) * B (K x N, ldb), FP32 row-major on the
+ * matrix cores (MFMA), FP32 accumulation.
+ * ------------------------------------------------------------------------- */
+int wotk_gemm_f32(const float *A, int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc,
+                  int64_t M, int64_t N, int64_t K, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Adjoint (no counterpart in the reference: the gradient of its primal
+ * objective, optimal_transport.py:45-53, at the converged plan).  Given the
+ * converged potentials f, g of a solve on C = sqeuclid(X, Y) / median, returns
+ * d OT / d X (I x d) and d OT / d log g (I) where G = g^dt
+ * (ot_model.py:314-315), by the envelope theorem with
+ * dP/dC_ij = (R_ij - exp(-C_ij/eps)) / (I J), including the median's
+ * subgradient through the middle element(s) median_index (may be NULL: the
+ * median is then treated as a constant).  grad_x or grad_log_g may be NULL.
+ * ------------------------------------------------------------------------- */
+size_t wotk_adjoint_workspace_size(int64_t I, int64_t J, int64_t d);
+int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int64_t ldct, int64_t I,
+                    int64_t J, const double *X, const double *Y, int64_t d, const double *G,
+                    double dt, const double *f, const double *g, double lambda1, double lambda2,
+                    double epsilon, const double *median, const int64_t *median_index,
+                    double *grad_x, double *grad_log_g, void *workspace, size_t workspace_bytes,
+                    void *stream);
 
 #ifdef __cplusplus
 }

@@ -17,13 +17,16 @@ check cadence):
 * ``transport_stablev2``            wot/ot/optimal_transport.py:167-236
 * ``default_cost_matrix``           wot/ot/ot_model.py:242-253 (sklearn
   ``pairwise_distances(metric='sqeuclidean')`` resolves to scipy ``cdist``)
-* ``glue`` / ``chain``              wot/tmap/util.py:74-94, wot/tmap/chaining.py:4-40
+* ``chain``                         wot/tmap/util.py:74-94, wot/tmap/chaining.py:4-40
+* ``objective_and_grad``            new (no reference counterpart): the converged
+  primal and its envelope-theorem gradient in the embeddings and log growth
 
 Pinning (how each function is checked independently of this file):
 golden vectors written from the reference file itself (tests/golden/), the
 library routines scipy.cdist / numpy.median, closed forms (I=J=1 fixed point,
-balanced-limit Sinkhorn), weak duality, and brute-force convex minimisation
-on tiny problems.  See tests/test_oracle.py.
+balanced-limit Sinkhorn), weak duality, brute-force convex minimisation on
+tiny problems, and central finite differences for the gradient.  See
+tests/test_oracle.py.
 """
 import logging
 import warnings
@@ -38,8 +41,8 @@ logger = logging.getLogger('wot')
 # --------------------------------------------------------------------------
 def compute_transport_matrix(solver, **params):
     """Growth-learning outer loop.  Iteration 0 solves with the given G,
-    iteration k>0 with the previous plan's row sums.  ``params['G']`` is
-    overwritten in the caller's dict, as in the reference (:28)."""
+    iteration k>0 with the previous plan's row sums (:28 rebinds ``G`` in the
+    function's own keyword dict; the caller's dict is not touched)."""
     growth = params['G']
     n_outer = params['growth_iters']
     learned = []
@@ -162,7 +165,7 @@ def optimal_transport_duality_gap(C, G, lambda1, lambda2, epsilon, batch_size, t
         raise RuntimeError("Overflow encountered in duality gap computation, please report this incident")
     if stats is not None:
         stats.update(iterations=n_iter, stabilisations=n_stab, gap=gap, epsilon_final=eps_i,
-                     max_iter_reached=False)
+                     max_iter_reached=False, u=u + eps_i * np.log(a), v=v + eps_i * np.log(b))
     return R / C.shape[1]
 
 
@@ -254,6 +257,65 @@ def growth_obs(learned_growth, tmap, delta_days):
     """obs columns g0..gN of the returned AnnData   (ot_model.py:319-325)."""
     lg = list(learned_growth) + [tmap.sum(axis=1)]
     return {'g' + str(i): np.power(g, 1.0 / delta_days) for i, g in enumerate(lg)}
+
+
+# --------------------------------------------------------------------------
+# converged objective and its gradient (no reference counterpart: the
+# gradient of primal(), optimal_transport.py:45-53, at the optimal plan)
+# --------------------------------------------------------------------------
+def middle_indices(D):
+    """Flat indices of the element(s) holding numpy.median's middle value(s):
+    the smallest index holding the lower / upper middle value."""
+    flat = np.ravel(D)
+    order = np.sort(flat)
+    n = flat.size
+    lo = order[(n - 1) // 2]
+    hi = order[n // 2]
+    return int(np.flatnonzero(flat == lo)[0]), int(np.flatnonzero(flat == hi)[0])
+
+
+def objective_and_grad(x, log_g, y, lambda1, lambda2, epsilon, dt=1.0, solver='fixed_iters', **params):
+    """OT objective = primal at the converged plan of C = sqeuclid(x, y)/median,
+    G = exp(dt log_g), and its gradient in (x, log_g) by the envelope theorem:
+    dP/dC_ij = (R_ij - exp(-C_ij/eps))/(IJ) chained through the cost (with the
+    median's subgradient) and through p and q = mean(p).  Returns
+    (objective, grad_x, grad_log_g, potentials dict)."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    D = sqeuclidean(x, y)
+    m = np.median(D)
+    C = D / m
+    G = np.exp(dt * np.asarray(log_g, dtype=np.float64))
+    n_i, n_j = C.shape
+    st = {}
+    if solver == 'fixed_iters':
+        transport_stablev2(C, lambda1, lambda2, epsilon, params.get('scaling_iter', 3000), G,
+                           params.get('tau', 1e4), params.get('epsilon0', 1.0), params.get('extra_iter', 1000),
+                           params.get('inner_iter_max', 50), stats=st)
+    else:
+        optimal_transport_duality_gap(C, G, lambda1, lambda2, epsilon, params.get('batch_size', 5),
+                                      params.get('tolerance', 1e-8), params.get('tau', 1e4),
+                                      params.get('epsilon0', 1.0), params.get('max_iter', 1e7), stats=st)
+    eps = st['epsilon_final']
+    f, g = st['u'], st['v']
+    R = np.exp((f[:, None] + g[None, :] - C) / eps)
+    Kt = np.exp(-C / eps)
+    dx, dy = np.ones(n_i) / n_i, np.ones(n_j) / n_j
+    q = np.ones(n_j) * np.average(G)
+    obj = primal(C, Kt, R, dx, dy, G, q, None, None, eps, lambda1, lambda2)
+    W = (R - Kt) / (n_i * n_j)
+    gx = (2.0 / m) * (x * W.sum(1)[:, None] - W.dot(y))
+    S = np.sum(W * C)
+    lo, hi = middle_indices(D)
+    mids = [(lo, 1.0)] if D.size % 2 else [(lo, 0.5), (hi, 0.5)]
+    for flat, w in mids:
+        k, l = divmod(flat, n_j)
+        gx[k] -= (S / m) * w * 2.0 * (x[k] - y[l])
+    xr = R.sum(1) / n_j
+    yc = R.sum(0) / n_i
+    dp = lambda1 / n_i * (1.0 - xr / G) + np.sum(lambda2 / n_j * (1.0 - yc / q)) / n_i
+    glg = dt * G * dp
+    return obj, gx, glg, dict(f=f, g=g, epsilon=eps, median=m, C=C)
 
 
 # --------------------------------------------------------------------------

@@ -204,3 +204,55 @@ def test_chain_integer_fixtures():
                     nxt[i, j] += bf[i, k] * m[k, j]
         bf = nxt
     assert np.array_equal(out, bf)
+
+
+def test_adjoint_gradient_matches_finite_differences():
+    """Envelope-theorem gradient of the converged objective (oracle) against
+    central finite differences of the converged FP64 objective, h = 1e-4."""
+    rng = np.random.default_rng(21)
+    n_i, n_j, d = 9, 11, 3
+    x = rng.standard_normal((n_i, d))
+    y = rng.standard_normal((n_j, d)) + 0.3
+    lg = 0.3 * rng.standard_normal(n_i)
+    prm = dict(lambda1=1.0, lambda2=5.0, epsilon=0.2, scaling_iter=3000, epsilon0=0.2, extra_iter=0,
+               inner_iter_max=50)
+
+    def obj(xx, ll):
+        return O.objective_and_grad(xx, ll, y, **prm)[0]
+
+    val, gx, glg, _ = O.objective_and_grad(x, lg, y, **prm)
+    h = 1e-4
+    for (i, k) in [(0, 0), (3, 1), (5, 2), (8, 0), (2, 2)]:
+        xp, xm = x.copy(), x.copy()
+        xp[i, k] += h
+        xm[i, k] -= h
+        fd = (obj(xp, lg) - obj(xm, lg)) / (2 * h)
+        assert fd == pytest.approx(gx[i, k], rel=1e-5, abs=1e-9), (i, k, fd, gx[i, k])
+    for i in [0, 4, 7]:
+        lp, lm = lg.copy(), lg.copy()
+        lp[i] += h
+        lm[i] -= h
+        fd = (obj(x, lp) - obj(x, lm)) / (2 * h)
+        assert fd == pytest.approx(glg[i], rel=1e-5, abs=1e-9), (i, fd, glg[i])
+
+
+def test_adjoint_median_subgradient_matters():
+    """Perturbing a middle element moves the median: the FD derivative of the
+    objective along that coordinate includes the median term."""
+    rng = np.random.default_rng(5)
+    n_i, n_j, d = 6, 8, 2
+    x = rng.standard_normal((n_i, d))
+    y = rng.standard_normal((n_j, d))
+    lg = np.zeros(n_i)
+    prm = dict(lambda1=1.0, lambda2=5.0, epsilon=0.3, scaling_iter=2000, epsilon0=0.3, extra_iter=0,
+               inner_iter_max=50)
+    D = O.sqeuclidean(x, y)
+    lo, hi = O.middle_indices(D)
+    k, _ = divmod(lo, n_j)
+    _, gx, _, _ = O.objective_and_grad(x, lg, y, **prm)
+    h = 1e-5
+    xp, xm = x.copy(), x.copy()
+    xp[k, 0] += h
+    xm[k, 0] -= h
+    fd = (O.objective_and_grad(xp, lg, y, **prm)[0] - O.objective_and_grad(xm, lg, y, **prm)[0]) / (2 * h)
+    assert fd == pytest.approx(gx[k, 0], rel=1e-5, abs=1e-10)

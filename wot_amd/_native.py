@@ -47,11 +47,11 @@ SIGNATURES = {
     'wotk_last_error': (ctypes.c_char_p, []),
     'wotk_abi_version': (ctypes.c_int, []),
     'wotk_cost_workspace_size': (_SZ, [_I64, _I64, _I64, _I32]),
-    'wotk_cost_sqeuclid': (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _I32, _P, _I64, _P, _I64, _P, _P,
+    'wotk_cost_sqeuclid': (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _I32, _P, _I64, _P, _I64, _P, _P, _P,
                                           _SZ, _P]),
     'wotk_sqeuclid_f64': (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _I64, _P]),
     'wotk_median_workspace_size': (_SZ, [_I64]),
-    'wotk_median_f64': (ctypes.c_int, [_P, _I64, _P, _P, _SZ, _P]),
+    'wotk_median_f64': (ctypes.c_int, [_P, _I64, _P, _P, _P, _SZ, _P]),
     'wotk_cost_import': (ctypes.c_int, [_P, _I64, _I64, _I64, _D, _P, _I64, _P, _I64, _P]),
     'wotk_solver_workspace_size': (_SZ, [_I64, _I64]),
     'wotk_transport_stablev2': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64, _D,
@@ -60,6 +60,10 @@ SIGNATURES = {
     'wotk_transport_duality_gap': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64,
                                                   _D, _D, _D, _I64, _P, _I64, _P, _P, _P, _P, _SZ,
                                                   ctypes.POINTER(Stats), _P]),
+    'wotk_gemm_f32': (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _I64, _I64, _I64, _P]),
+    'wotk_adjoint_workspace_size': (_SZ, [_I64, _I64, _I64]),
+    'wotk_ot_adjoint': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _P, _I64, _P, _D, _P, _P, _D, _D, _D,
+                                       _P, _P, _P, _P, _P, _SZ, _P]),
     'wotk_primal_dual': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _P, _P, _D, _D, _D,
                                         ctypes.POINTER(ctypes.c_double), _P, _SZ, _P]),
 }

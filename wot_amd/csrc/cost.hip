@@ -81,6 +81,7 @@ struct SelectState {
     unsigned long long eq;     // after the last pass: count of keys == prefix
     unsigned long long next;   // smallest key > prefix (for the upper median)
     double median;
+    unsigned long long lo_key, hi_key;  // the two middle values (equal for odd n)
 };
 
 constexpr int kHistBins = 2048;
@@ -177,7 +178,30 @@ __global__ void median_finish_kernel(SelectState *st, int odd, double *out) {
         m = (lo + hi) / 2.0;
     }
     st->median = m;
+    st->lo_key = st->prefix;
+    st->hi_key = odd ? st->prefix
+                     : (((unsigned long long)(st->rank + 1) < st->eq) ? st->prefix : st->next);
     if (out) *out = m;
+}
+
+// Flat indices of (one of) the elements holding the lower / upper middle
+// value: the smallest index with that key.  These carry the median's
+// subgradient (weight 1/2 each for an even count, 1 for an odd count).
+__global__ void median_index_init_kernel(unsigned long long *idx) {
+    idx[0] = ~0ull;
+    idx[1] = ~0ull;
+}
+
+__global__ __launch_bounds__(256) void median_index_kernel(const unsigned long long *__restrict__ keys,
+                                                           int64_t n, const SelectState *__restrict__ st,
+                                                           unsigned long long *__restrict__ idx) {
+    const unsigned long long lo = st->lo_key, hi = st->hi_key;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+        const unsigned long long k = keys[e];
+        if (k == lo) atomicMin(&idx[0], (unsigned long long)e);
+        if (k == hi) atomicMin(&idx[1], (unsigned long long)e);
+    }
 }
 
 static const int kDigitShift[6] = {53, 42, 31, 20, 10, 0};
@@ -190,8 +214,8 @@ static size_t median_ws(Arena &ar, int64_t n, SelectState **st, unsigned long lo
     return ar.off;
 }
 
-static int median_run(const double *values, int64_t n, double *out, SelectState *st,
-                      unsigned long long *hist, hipStream_t s) {
+static int median_run(const double *values, int64_t n, double *out, int64_t *index_out,
+                      SelectState *st, unsigned long long *hist, hipStream_t s) {
     WOTK_REQUIRE(n > 0, "median of an empty array");
     const long long rank = (n % 2) ? n / 2 : n / 2 - 1;
     median_init_kernel<<<1, 256, 0, s>>>(st, rank, hist);
@@ -211,6 +235,12 @@ static int median_run(const double *values, int64_t n, double *out, SelectState 
     }
     median_finish_kernel<<<1, 1, 0, s>>>(st, (int)(n % 2), out);
     WOTK_LAUNCH_CHECK();
+    if (index_out) {
+        auto *idx = reinterpret_cast<unsigned long long *>(index_out);
+        median_index_init_kernel<<<1, 1, 0, s>>>(idx);
+        median_index_kernel<<<grid, 256, 0, s>>>(keys, n, st, idx);
+        WOTK_LAUNCH_CHECK();
+    }
     return WOTK_OK;
 }
 
@@ -289,14 +319,15 @@ extern "C" size_t wotk_median_workspace_size(int64_t n) {
     return median_ws(ar, n, &st, &h);
 }
 
-extern "C" int wotk_median_f64(const double *values, int64_t n, double *median_out, void *workspace,
-                               size_t workspace_bytes, void *stream) {
+extern "C" int wotk_median_f64(const double *values, int64_t n, double *median_out,
+                               int64_t *median_index_out, void *workspace, size_t workspace_bytes,
+                               void *stream) {
     Arena ar(workspace, workspace_bytes);
     SelectState *st;
     unsigned long long *h;
     median_ws(ar, n, &st, &h);
     if (!ar.ok || !workspace) return fail(WOTK_EWORKSPACE, "median workspace too small");
-    return median_run(values, n, median_out, st, h, (hipStream_t)stream);
+    return median_run(values, n, median_out, median_index_out, st, h, (hipStream_t)stream);
 }
 
 extern "C" size_t wotk_cost_workspace_size(int64_t I, int64_t J, int64_t d, int32_t mode) {
@@ -318,8 +349,8 @@ extern "C" int wotk_sqeuclid_f64(const double *X, const double *Y, int64_t I, in
 
 extern "C" int wotk_cost_sqeuclid(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
                                   int32_t mode, float *C, int64_t ldc, float *CT, int64_t ldct,
-                                  double *median_out, void *workspace, size_t workspace_bytes,
-                                  void *stream) {
+                                  double *median_out, int64_t *median_index_out, void *workspace,
+                                  size_t workspace_bytes, void *stream) {
     int rc = check_cost_layout(I, J, d, C, ldc, CT, ldct);
     if (rc) return rc;
     WOTK_REQUIRE(mode == WOTK_COST_F64 || mode == WOTK_COST_BF16X3, "unknown cost mode %d", mode);
@@ -335,7 +366,7 @@ extern "C" int wotk_cost_sqeuclid(const double *X, const double *Y, int64_t I, i
     rc = (mode == WOTK_COST_F64) ? launch_sqeuclid_f64(X, Y, I, J, d, C64, J, s)
                                  : launch_cost_bf16x3(X, Y, I, J, d, C64, J, scratch, s);
     if (rc) return rc;
-    rc = median_run(C64, I * J, nullptr, st, h, s);
+    rc = median_run(C64, I * J, nullptr, median_index_out, st, h, s);
     if (rc) return rc;
     rc = launch_normalise_transpose(C64, J, I, J, &st->median, 1.0, C, ldc, CT, ldct, s);
     if (rc) return rc;

@@ -26,8 +26,8 @@ def write_dataset(ds, path, output_format='txt'):
         ds.write(path)
     elif output_format == 'npz':
         obs = {('obs_' + c): ds.obs[c].values for c in ds.obs.columns}
-        np.savez(path, X=np.asarray(ds.X), obs_index=np.asarray(ds.obs.index.astype(str)),
-                 var_index=np.asarray(ds.var.index.astype(str)), **obs)
+        np.savez(path, X=np.asarray(ds.X), obs_index=np.array([str(v) for v in ds.obs.index], dtype=str),
+                 var_index=np.array([str(v) for v in ds.var.index], dtype=str), **obs)
     else:
         raise ValueError('Unknown file format')
     return path

@@ -1,4 +1,5 @@
 """OT solvers and the OTModel driver (the wot.ot namespace of the reference)."""
+from .adjoint import ot_objective  # noqa: F401
 from .device import DeviceCost  # noqa: F401
 from .initializer import (parse_configuration, parse_parameter_file,  # noqa: F401
                           parse_per_timepair_configuration, parse_per_timepoint_configuration)

@@ -48,11 +48,12 @@ class DeviceCost:
     """FP32 cost pair (C, C^T) on the device, plus its FP64 median when it was
     computed from coordinates (OTModel.compute_default_cost_matrix semantics)."""
 
-    def __init__(self, C, CT, n_i, n_j, median=None):
+    def __init__(self, C, CT, n_i, n_j, median=None, median_index=None):
         self.C, self.CT = C, CT
         self.n_i, self.n_j = int(n_i), int(n_j)
         self.ldc, self.ldct = C.shape[1], CT.shape[1]
-        self.median = median  # device scalar tensor or None
+        self.median = median              # device scalar tensor or None
+        self.median_index = median_index  # device int64[2]: flat indices of the middle element(s)
 
     @property
     def shape(self):
@@ -83,9 +84,11 @@ class DeviceCost:
         ws = workspace(torch, L.wotk_cost_workspace_size(n_i, n_j, d, m), device)
         C, CT = cls._alloc(torch, n_i, n_j, device)
         med = torch.empty(1, dtype=torch.float64, device=device)
+        midx = torch.empty(2, dtype=torch.int64, device=device)
         _native.check(L.wotk_cost_sqeuclid(_ptr(X), _ptr(Y), n_i, n_j, d, m, _ptr(C), C.shape[1], _ptr(CT),
-                                           CT.shape[1], _ptr(med), _ptr(ws), ws.numel(), _stream(torch)))
-        return cls(C, CT, n_i, n_j, median=med)
+                                           CT.shape[1], _ptr(med), _ptr(midx), _ptr(ws), ws.numel(),
+                                           _stream(torch)))
+        return cls(C, CT, n_i, n_j, median=med, median_index=midx)
 
     @classmethod
     def from_matrix(cls, cost, scale=1.0, device='cuda'):
@@ -123,15 +126,18 @@ def sqeuclid_f64(x, y, device='cuda'):
     return out
 
 
-def median_f64(values):
-    """Exact numpy-style median of a float64 device tensor (all values >= 0)."""
+def median_f64(values, return_index=False):
+    """Exact numpy-style median of a float64 device tensor (all values >= 0);
+    with ``return_index`` also the flat indices of the two middle elements."""
     torch = torch_mod()
     v = values.contiguous().view(-1)
     L = _native.lib()
     ws = workspace(torch, L.wotk_median_workspace_size(v.numel()), v.device)
     out = torch.empty(1, dtype=torch.float64, device=v.device)
-    _native.check(L.wotk_median_f64(_ptr(v), v.numel(), _ptr(out), _ptr(ws), ws.numel(), _stream(torch)))
-    return out
+    idx = torch.empty(2, dtype=torch.int64, device=v.device)
+    _native.check(L.wotk_median_f64(_ptr(v), v.numel(), _ptr(out), _ptr(idx), _ptr(ws), ws.numel(),
+                                    _stream(torch)))
+    return (out, idx) if return_index else out
 
 
 def solve(cost, G, solver, params, want_plan=True, want_row_sums=False, want_potentials=False):

@@ -1,0 +1,66 @@
+"""Long-range couplings on the device (wot.tmap.glue_transport_maps /
+chain_transport_maps, wot/tmap/util.py:74-94 and wot/tmap/chaining.py:4-40).
+
+Maps are multiplied in FP32 on the matrix cores (``wotk_gemm_f32``) and kept
+in HBM between products; as in the reference there is no normalisation.
+"""
+import ctypes
+
+import numpy as np
+
+from .. import _native, _anndata
+from ..ot import device as _dev
+
+
+def gemm_f32(A, B, out=None):
+    """A (M x K) @ B (K x N) in FP32 on the device (torch tensors or arrays)."""
+    torch = _dev.torch_mod()
+    A = (A if isinstance(A, torch.Tensor) else torch.from_numpy(np.asarray(A))).to('cuda', torch.float32)
+    B = (B if isinstance(B, torch.Tensor) else torch.from_numpy(np.asarray(B))).to('cuda', torch.float32)
+    A = A.contiguous()
+    B = B.contiguous()
+    if A.ndim != 2 or B.ndim != 2 or A.shape[1] != B.shape[0]:
+        raise ValueError('shape mismatch %s @ %s' % (tuple(A.shape), tuple(B.shape)))
+    M, K = A.shape
+    N = B.shape[1]
+    C = out if out is not None else torch.empty((M, N), dtype=torch.float32, device=A.device)
+    p = _dev._ptr
+    _native.check(_native.lib().wotk_gemm_f32(p(A), A.stride(0), p(B), B.stride(0), p(C), C.stride(0), M, N, K,
+                                              ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return C
+
+
+def chain_product(maps):
+    """T_0 T_1 ... T_n (FP32, device); returns a torch tensor."""
+    out = None
+    for m in maps:
+        out = m if out is None else gemm_f32(out, m)
+    torch = _dev.torch_mod()
+    return out if isinstance(out, torch.Tensor) else torch.from_numpy(np.asarray(out, np.float32)).cuda()
+
+
+def glue_transport_maps(tmap_0, tmap_1):
+    """Transport map t0 -> t2 from t0 -> t1 and t1 -> t2 (util.py:74-94): rows
+    of tmap_1 re-ordered to tmap_0's columns, then multiplied (no column
+    normalisation, as in the reference)."""
+    idx = tmap_1.obs.index.get_indexer_for(tmap_0.var.index)
+    x1 = np.asarray(tmap_1.X)[idx, :]
+    res = gemm_f32(np.asarray(tmap_0.X, np.float32), x1.astype(np.float32)).cpu().numpy().astype(np.float64)
+    return _anndata.AnnData(res, tmap_0.obs.copy(), tmap_1.var.copy())
+
+
+def chain_transport_maps(tmap_model, pairs_list):
+    """Chain the maps of consecutive pairs (chaining.py:4-40); ``tmap_model``
+    needs ``get_coupling(t0, t1)`` returning an AnnData-like map."""
+    pairs_list = list(pairs_list)
+    for i in range(len(pairs_list) - 1):
+        if pairs_list[i][1] != pairs_list[i + 1][0]:
+            raise ValueError("Transport maps cannot be glued together : invalid path")
+    for a, b in pairs_list:
+        if a >= b:
+            raise ValueError("({}, {}) is not a valid transport map : it goes backwards in time".format(a, b))
+    tmap_0 = tmap_model.get_coupling(*pairs_list.pop(0))
+    while len(pairs_list) > 0:
+        tmap_1 = tmap_model.get_coupling(*pairs_list.pop(0))
+        tmap_0 = glue_transport_maps(tmap_0, tmap_1)
+    return tmap_0
