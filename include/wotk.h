@@ -126,6 +126,42 @@ int wotk_transport_duality_gap(const float *C, int64_t ldc, const float *CT, int
                                double *g_out, void *workspace, size_t workspace_bytes,
                                wotk_stats *stats, void *stream);
 
+/* The plan exp((f_i + g_j - C_ij)/eps) / divisor (I x J, ldp, FP64) and/or its
+ * row sums from potentials: optimal_transport.py:234-236 (R / J) and :145
+ * (divisor 1).  plan or row_sums may be NULL. */
+int wotk_plan(const float *C, int64_t ldc, int64_t I, int64_t J, const double *f, const double *g,
+              double epsilon, double divisor, double *plan, int64_t ldp, double *row_sums,
+              void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Batched fixed-iteration solves: every day pair of a time course
+ * (ot_model.py:124-201 solves them one after another) times n_combos
+ * (epsilon, lambda1) combinations (the reference sweeps a grid as one process
+ * per combination), transport_stablev2 semantics for each solve.  All costs
+ * live in one FP32 buffer; pair p's C_p and C_p^T start at c_offset /
+ * ct_offset (multiples of 4 elements).  G holds the concatenated growth
+ * vectors.  pairs, epsilon and lambda1 are HOST arrays.  f_out / g_out
+ * (device, may be NULL) receive the final potentials as consecutive blocks
+ * [pair][combination][I_p] (resp. [J_p]).  graph_iters > 0 captures that many
+ * iterations in a hipGraph once and replays it.
+ * ------------------------------------------------------------------------- */
+typedef struct wotk_batch_pair {
+    int64_t c_offset;   /* element offset of C_p (I_p x J_p, ldc) in the cost buffer */
+    int64_t ct_offset;  /* element offset of C_p^T (J_p x I_p, ldct)                 */
+    int64_t ldc, ldct;
+    int64_t n_i, n_j;
+    int64_t g_offset;   /* element offset of G_p (I_p) in the growth buffer         */
+} wotk_batch_pair;
+
+size_t wotk_batch_workspace_size(const wotk_batch_pair *pairs, int32_t n_pairs, int32_t n_combos,
+                                 int64_t n_iterations);
+int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pairs, int32_t n_pairs,
+                        const double *G, int32_t n_combos, const double *epsilon,
+                        const double *lambda1, double lambda2, int64_t scaling_iter, double tau,
+                        double epsilon0, int64_t extra_iter, int64_t inner_iter_max, double *f_out,
+                        double *g_out, int32_t graph_iters, void *workspace,
+                        size_t workspace_bytes, wotk_stats *stats, void *stream);
+
 /* Primal / dual objectives of optimal_transport.py:45-62 at the plan
  * R = exp((f_i + g_j - C_ij)/eps) (un-normalised), FP64 reductions.
  * out[0] = primal, out[1] = dual (device or host pointer to 2 doubles: host). */

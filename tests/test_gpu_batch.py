@@ -1,0 +1,83 @@
+"""Batched time course (all day pairs x an (epsilon, lambda1) grid in one
+persistent sweep per half-iteration) against the FP64 oracle, solve by solve."""
+import numpy as np
+import pytest
+
+from oracle import ot_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SCHED = dict(scaling_iter=300, extra_iter=100, inner_iter_max=50, tau=1e4)
+
+
+@pytest.fixture(scope='module')
+def batch():
+    import torch
+    assert torch.cuda.is_available()
+    from wot_amd.ot import batch
+    return batch
+
+
+def _course(seed, counts, dim=10):
+    from wot_amd.synthetic import time_course
+    coords, growth, _ = time_course(dim=dim, n_components=4, seed=seed, counts=counts)
+    return coords, growth
+
+
+def _check(res, coords, growth, epsilon0, lambda2=50.0, tol=1e-5):
+    worst = 0.0
+    for p in range(len(coords) - 1):
+        C = O.default_cost_matrix(coords[p], coords[p + 1])
+        for e, (eps, lam1) in enumerate(res.combos):
+            R = res.plan(p, e)
+            Rref = O.transport_stablev2(C, lam1, lambda2, eps, G=growth[p], epsilon0=epsilon0, **SCHED)
+            err = np.linalg.norm(R - Rref) / np.linalg.norm(Rref)
+            marg = max(np.abs(R.sum(a) - Rref.sum(a)).max() / np.abs(Rref.sum(a)).max() for a in (0, 1))
+            worst = max(worst, err)
+            assert err <= tol and marg <= 1e-6, (p, eps, lam1, err, marg)
+    return worst
+
+
+def test_batch_grid_matches_oracle(batch):
+    coords, growth = _course(1, [150, 320, 200, 410])
+    res = batch.solve_time_course(coords, growth, epsilons=(0.01, 0.05, 0.1), lambda1s=(0.5, 1.0, 5.0),
+                                  graph_iters=50, **SCHED)
+    assert len(res.combos) == 9
+    worst = _check(res, coords, growth, 1.0)
+    print('batch worst rel frob %.3e' % worst)
+
+
+def test_batch_graph_replay_is_deterministic(batch):
+    coords, growth = _course(2, [130, 257, 96])
+    kw = dict(epsilons=(0.05, 0.1), lambda1s=(1.0, 5.0), **SCHED)
+    a = batch.solve_time_course(coords, growth, graph_iters=50, **kw)
+    b = batch.solve_time_course(coords, growth, graph_iters=0, **kw)
+    c = batch.solve_time_course(coords, growth, graph_iters=7, **kw)  # 400 = 57*7 + 1: replays + a remainder
+    for p in range(2):
+        assert np.array_equal(a.f[p].cpu().numpy(), b.f[p].cpu().numpy())
+        assert np.array_equal(a.g[p].cpu().numpy(), c.g[p].cpu().numpy())
+
+
+def test_batch_fallback_rows(batch):
+    """epsilon0 = eps = 0.01 with an outlier source cell: its first-iteration
+    terms all underflow FP32 under the fixed shift, so the online-maximum pass
+    must take over (FP64 stays finite: min C / eps < 700)."""
+    coords, growth = _course(3, [120, 140])
+    coords[0] = coords[0].copy()
+    coords[0][0] += 8.0  # far from every target cell
+    res = batch.solve_time_course(coords, growth, epsilons=(0.01,), lambda1s=(1.0,), epsilon0=0.01,
+                                  graph_iters=0, **SCHED)
+    C = O.default_cost_matrix(coords[0], coords[1])
+    assert C[0].min() / 0.01 * 1.4427 > 149 and C[0].min() / 0.01 < 700
+    _check(res, coords, growth, 0.01)
+
+
+def test_batch_single_combo_equals_single_solver(batch):
+    """E = 1 batch == the single-pair solver on the same cost."""
+    import wot_amd.ot as ot
+    coords, growth = _course(4, [200, 230])
+    res = batch.solve_time_course(coords, growth, epsilons=(0.05,), lambda1s=(1.0,), graph_iters=0, **SCHED)
+    cost = ot.DeviceCost.from_coordinates(coords[0], coords[1])
+    R1 = ot.transport_stablev2(cost, G=growth[0], lambda1=1.0, lambda2=50.0, epsilon=0.05, epsilon0=1.0, **SCHED)
+    R2 = res.plan(0, 0)
+    assert np.linalg.norm(R1 - R2) / np.linalg.norm(R1) < 1e-6

@@ -184,3 +184,21 @@ def test_unbalanced_column_marginal_identity(ot):
     got = R.sum(0)
     err = ((got - want).abs().max() / want.abs().max()).item()
     assert err < 1e-6, err
+
+
+def test_cost_bf16x3_error_is_bounded(ot):
+    """Split-BF16 (3-term) MFMA cost: the median-normalised cost stays within the
+    error the split allows (missing x_lo.y_lo ~ 2^-16 of |x||y| per term) and
+    its median is exact for the values it produced."""
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(2000, 2500, dim=30, seed=0)
+    exact = ot.DeviceCost.from_coordinates(dp.x, dp.y)
+    approx = ot.DeviceCost.from_coordinates(dp.x, dp.y, mode='bf16x3')
+    a = approx.dense().cpu().numpy().astype(np.float64)
+    e = exact.dense().cpu().numpy().astype(np.float64)
+    err = np.abs(a - e)
+    near = e <= np.quantile(e, 0.01)
+    print('bf16x3 cost error: max %.3e rms %.3e near-rms %.3e' % (err.max(), np.sqrt((err ** 2).mean()),
+                                                                  np.sqrt((err[near] ** 2).mean())))
+    assert err.max() < 5e-5 and np.sqrt((err ** 2).mean()) < 5e-6
+    assert np.array_equal(approx.CT[:, :2000].cpu().numpy(), approx.dense().cpu().numpy().T)

@@ -60,6 +60,10 @@ SIGNATURES = {
     'wotk_transport_duality_gap': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64,
                                                   _D, _D, _D, _I64, _P, _I64, _P, _P, _P, _P, _SZ,
                                                   ctypes.POINTER(Stats), _P]),
+    'wotk_plan': (ctypes.c_int, [_P, _I64, _I64, _I64, _P, _P, _D, _D, _P, _I64, _P, _P]),
+    'wotk_batch_workspace_size': (_SZ, [_P, _I32, _I32, _I64]),
+    'wotk_batch_stablev2': (ctypes.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _D, _I64, _D, _D, _I64, _I64, _P, _P,
+                                           _I32, _P, _SZ, ctypes.POINTER(Stats), _P]),
     'wotk_gemm_f32': (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _I64, _I64, _I64, _P]),
     'wotk_adjoint_workspace_size': (_SZ, [_I64, _I64, _I64]),
     'wotk_ot_adjoint': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _P, _I64, _P, _D, _P, _P, _D, _D, _D,
@@ -67,6 +71,12 @@ SIGNATURES = {
     'wotk_primal_dual': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _P, _P, _D, _D, _D,
                                         ctypes.POINTER(ctypes.c_double), _P, _SZ, _P]),
 }
+
+
+class BatchPair(ctypes.Structure):
+    _fields_ = [('c_offset', ctypes.c_int64), ('ct_offset', ctypes.c_int64), ('ldc', ctypes.c_int64),
+                ('ldct', ctypes.c_int64), ('n_i', ctypes.c_int64), ('n_j', ctypes.c_int64),
+                ('g_offset', ctypes.c_int64)]
 
 
 class NativeError(RuntimeError):

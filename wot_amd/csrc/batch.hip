@@ -1,0 +1,552 @@
+// Batched fixed-iteration solves (transport_stablev2 semantics,
+// wot/ot/optimal_transport.py:167-236) for a whole time course: every day pair
+// (wot/ot/ot_model.py:124-201 solves them one at a time) times a grid of
+// (epsilon, lambda1) combinations (the reference's parameter sweep is one
+// process per combination, workflow/optimal_transport_validation_parameter_sweep.wdl).
+//
+// One launch per half-iteration sweeps all pairs and all combinations: a
+// persistent grid pulls (pair, row-block) tiles from a work queue ordered
+// longest-row first, and every cost entry read from HBM serves all E
+// combinations.  Potentials are stored combination-minor ([n][E]) so a lane
+// reads the E values of its columns contiguously.  The per-iteration
+// epsilon / alpha / k table lives in device memory and the kernels read the
+// iteration index from a device counter, so a block of iterations can be
+// captured once in a hipGraph and replayed.
+//
+// Exponents use the row's own previous potential as a fixed shift:
+//   y_ij = (f_i_old + h_j - C_ij) * log2(e)/eps,   S_i = sum_j 2^y_ij,
+//   LSE_i = ln S_i - f_i_old/eps,
+// (the terms are then plan entries, bounded by the row mass), with an
+// online-maximum recomputation for any row whose sum under- or overflows.
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+
+namespace wotk {
+
+constexpr int kMaxCombos = 16;
+constexpr int kBThreads = 256;
+constexpr int kBRows = 2;
+
+struct PairDev {
+    long long a_off;   // element offset of this pair's matrix (C or C^T) in the cost buffer
+    long long lda;
+    int nrows, ncols;
+    long long pot_off;    // offset of this side's potentials ([nrows][E]) in the pot buffer
+    long long other_off;  // offset of the other side's potentials ([ncols pad 4][E])
+    long long logw_off;   // offset of log weights ([nrows]) or -1 for the pair's scalar
+    int pair;
+};
+
+struct IterParam {  // one (iteration, combination)
+    double k;       // log2(e) / eps
+    double eps;
+    double alpha1, alpha2;
+};
+
+struct Tile {
+    int side_pair;  // index into the PairDev table of this side
+    int row0;
+};
+
+struct BatchArgs {
+    const float *cost;
+    const PairDev *pairs;
+    const Tile *tiles;
+    int ntiles;
+    const IterParam *table;    // [iteration][E]
+    const int *iter;           // device iteration counter
+    int E;
+    int side;                  // 0: f from g over C (alpha1), 1: g from f over C^T (alpha2)
+    double *pot;               // this side's potentials
+    const double *other;       // the other side's potentials
+    const double *logw;        // per-row log weights (source side)
+    const double *logq;        // per-pair log q (target side)
+    const double *log_n_other; // per side-pair log(#columns)
+    int *queue;                // work-queue counter
+};
+
+__device__ __forceinline__ float bexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+__device__ __forceinline__ float4 bload4(const float *p) {
+    float4 v;
+    v.x = __builtin_nontemporal_load(p);
+    v.y = __builtin_nontemporal_load(p + 1);
+    v.z = __builtin_nontemporal_load(p + 2);
+    v.w = __builtin_nontemporal_load(p + 3);
+    return v;
+}
+
+template <int E>
+__global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
+    constexpr int NW = kBThreads / 64;
+    constexpr int RE = kBRows * E;
+    __shared__ int tile_sh;
+    __shared__ int bad_sh;
+    __shared__ double red_a[RE][NW];
+    __shared__ double red_b[RE][NW];
+    __shared__ double l2_sh[RE];  // log2 sum_j 2^((h_j - C_ij) k) per (row, combination)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int it = *ba.iter;
+    const IterParam *prm = ba.table + (size_t)it * E;
+    double kk[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) kk[e] = prm[e].k;
+
+    for (;;) {
+        if (tid == 0) tile_sh = atomicAdd(ba.queue, 1);
+        __syncthreads();
+        const int t = tile_sh;
+        if (tid == 0) bad_sh = 0;
+        __syncthreads();
+        if (t >= ba.ntiles) break;
+        const Tile tl = ba.tiles[t];
+        const PairDev pd = ba.pairs[tl.side_pair];
+        const float *A = ba.cost + pd.a_off;
+        const double *h = ba.other + pd.other_off;
+        double *pot = ba.pot + pd.pot_off;
+        const float *rowp[kBRows];
+        double fk[kBRows][E];
+        float s[kBRows][E];
+#pragma unroll
+        for (int r = 0; r < kBRows; ++r) {
+            const int row = min(tl.row0 + r, pd.nrows - 1);
+            rowp[r] = A + (long long)row * pd.lda;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                fk[r][e] = pot[(long long)row * E + e] * kk[e];
+                s[r][e] = 0.f;
+            }
+        }
+        // main pass: fixed shift f_old * k, one exp2 per (entry, combination)
+        for (int c = tid * 4; c < pd.ncols; c += kBThreads * 4) {
+            double cv[kBRows][4];
+#pragma unroll
+            for (int r = 0; r < kBRows; ++r) {
+                const float4 a = bload4(rowp[r] + c);
+                cv[r][0] = a.x;
+                cv[r][1] = a.y;
+                cv[r][2] = a.z;
+                cv[r][3] = a.w;
+            }
+            const double *hc = h + (long long)c * E;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                double hk[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) hk[q] = hc[q * E + e] * kk[e];
+#pragma unroll
+                for (int r = 0; r < kBRows; ++r) {
+                    float acc = 0.f;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc += bexp2((float)fma(-kk[e], cv[r][q], hk[q] + fk[r][e]));
+                    s[r][e] += acc;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kBRows; ++r)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                double v = (double)s[r][e];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (lane == 0) red_a[r * E + e][wave] = v;
+            }
+        __syncthreads();
+        if (tid < RE) {
+            double S = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) S += red_a[tid][w];
+            const int r = tid / E, e = tid % E;
+            l2_sh[tid] = log2(S) - fk[r][e];
+            if ((S == 0.0 || isinf(S)) && tl.row0 + r < pd.nrows) atomicOr(&bad_sh, 1);
+        }
+        __syncthreads();
+        if (bad_sh) {
+            // rare: the fixed shift under- or overflowed a row; recompute every
+            // (row, combination) of the tile with a per-lane online maximum
+            double sig[kBRows][E];
+#pragma unroll
+            for (int r = 0; r < kBRows; ++r)
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    sig[r][e] = -1.0e6;
+                    s[r][e] = 0.f;
+                }
+            for (int c = tid * 4; c < pd.ncols; c += kBThreads * 4) {
+                const double *hc = h + (long long)c * E;
+#pragma unroll
+                for (int r = 0; r < kBRows; ++r) {
+                    const float4 a = bload4(rowp[r] + c);
+                    const double av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+                    for (int e = 0; e < E; ++e)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const double y = fma(-kk[e], av[q], hc[q * E + e] * kk[e]);
+                            if (y > sig[r][e]) {
+                                const double ns = ceil(y);
+                                s[r][e] = ldexpf(s[r][e], (int)(sig[r][e] - ns));
+                                sig[r][e] = ns;
+                            }
+                            s[r][e] += bexp2((float)(y - sig[r][e]));
+                        }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kBRows; ++r)
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    double sg = sig[r][e], sd = (double)s[r][e];
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) {
+                        const double og = __shfl_xor(sg, off), os = __shfl_xor(sd, off);
+                        const double ng = fmax(sg, og);
+                        sd = ldexp(sd, (int)(sg - ng)) + ldexp(os, (int)(og - ng));
+                        sg = ng;
+                    }
+                    if (lane == 0) {
+                        red_a[r * E + e][wave] = sg;
+                        red_b[r * E + e][wave] = sd;
+                    }
+                }
+            __syncthreads();
+            if (tid < RE) {
+                double sg = red_a[tid][0], sd = red_b[tid][0];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) {
+                    const double og = red_a[tid][w], os = red_b[tid][w];
+                    const double ng = fmax(sg, og);
+                    sd = ldexp(sd, (int)(sg - ng)) + ldexp(os, (int)(og - ng));
+                    sg = ng;
+                }
+                l2_sh[tid] = sg + log2(sd);
+            }
+            __syncthreads();
+        }
+        if (tid < RE) {
+            const int r = tid / E, e = tid % E;
+            const int row = tl.row0 + r;
+            if (row < pd.nrows) {
+                const IterParam ip = prm[e];
+                const double lse = l2_sh[tid] * kLn2;  // ln sum_j exp((h_j - C_ij)/eps)
+                const double lw = (pd.logw_off >= 0) ? ba.logw[pd.logw_off + row] : ba.logq[pd.pair];
+                const double alpha = ba.side == 0 ? ip.alpha1 : ip.alpha2;
+                pot[(long long)row * E + e] = alpha * ip.eps * (lw + ba.log_n_other[tl.side_pair] - lse);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// advance the iteration counter and reset the two work queues (one thread)
+__global__ void batch_advance_kernel(int *iter, int *queues) {
+    iter[0] += 1;
+    queues[0] = 0;
+    queues[1] = 0;
+}
+
+__global__ void batch_reset_kernel(int *iter, int *queues, int start) {
+    iter[0] = start;
+    queues[0] = 0;
+    queues[1] = 0;
+}
+
+__global__ void batch_fill_kernel(double *x, long long n, double v) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+// log weights of the source rows and log q = log(mean G) per pair
+__global__ __launch_bounds__(256) void batch_growth_kernel(const double *__restrict__ G, const long long *__restrict__ g_off,
+                                                           const int *__restrict__ n_src, double *__restrict__ logw,
+                                                           const long long *__restrict__ lw_off, double *__restrict__ logq) {
+    const int p = blockIdx.x;
+    const double *g = G + g_off[p];
+    double *lw = logw + lw_off[p];
+    const int n = n_src[p];
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        lw[i] = log(g[i]);
+        acc += g[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) logq[p] = log((red[0] + red[1] + red[2] + red[3]) / n);
+}
+
+// potentials of one (pair, combination) from [n][E] to a contiguous vector
+__global__ void batch_gather_kernel(const double *__restrict__ pot, int n, int E, int e, double *__restrict__ out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = pot[(long long)i * E + e];
+}
+
+struct BatchWork {
+    double *f, *g, *logw, *logq, *lognf, *logng;
+    IterParam *table;
+    PairDev *pairs_f, *pairs_g;
+    Tile *tiles_f, *tiles_g;
+    long long *g_off, *lw_off;
+    int *n_src;
+    int *iter, *queues;
+};
+
+}  // namespace wotk
+
+using namespace wotk;
+
+static size_t batch_carve(Arena &ar, const wotk_batch_pair *pairs, int n_pairs, int E, int64_t n_iter,
+                          int64_t *nf_out, int64_t *ng_out, BatchWork *w, int64_t *ntf, int64_t *ntg) {
+    int64_t nf = 0, ng = 0, nlw = 0, tf = 0, tg = 0;
+    for (int p = 0; p < n_pairs; ++p) {
+        nf += round_up(pairs[p].n_i, 4);
+        ng += round_up(pairs[p].n_j, 4);
+        nlw += pairs[p].n_i;
+        tf += (pairs[p].n_i + kBRows - 1) / kBRows;
+        tg += (pairs[p].n_j + kBRows - 1) / kBRows;
+    }
+    w->f = ar.take<double>((size_t)nf * E);
+    w->g = ar.take<double>((size_t)ng * E);
+    w->logw = ar.take<double>(nlw);
+    w->logq = ar.take<double>(n_pairs);
+    w->lognf = ar.take<double>(n_pairs);
+    w->logng = ar.take<double>(n_pairs);
+    w->table = ar.take<IterParam>((size_t)(n_iter + 1) * E);
+    w->pairs_f = ar.take<PairDev>(n_pairs);
+    w->pairs_g = ar.take<PairDev>(n_pairs);
+    w->tiles_f = ar.take<Tile>(tf);
+    w->tiles_g = ar.take<Tile>(tg);
+    w->g_off = ar.take<long long>(n_pairs);
+    w->lw_off = ar.take<long long>(n_pairs);
+    w->n_src = ar.take<int>(n_pairs);
+    w->iter = ar.take<int>(4);
+    w->queues = ar.take<int>(4);
+    *nf_out = nf;
+    *ng_out = ng;
+    *ntf = tf;
+    *ntg = tg;
+    return ar.off;
+}
+
+extern "C" size_t wotk_batch_workspace_size(const wotk_batch_pair *pairs, int32_t n_pairs, int32_t n_combos,
+                                            int64_t n_iterations) {
+    Arena ar(nullptr, 0);
+    BatchWork w;
+    int64_t a, b, c, d;
+    return batch_carve(ar, pairs, n_pairs, n_combos, n_iterations, &a, &b, &w, &c, &d);
+}
+
+template <int E>
+static int launch_sweeps(const BatchArgs &af, const BatchArgs &ag, int grid_f, int grid_g, int *iter, int *queues,
+                         hipStream_t s) {
+    batch_sweep_kernel<E><<<grid_f, kBThreads, 0, s>>>(af);
+    batch_sweep_kernel<E><<<grid_g, kBThreads, 0, s>>>(ag);
+    batch_advance_kernel<<<1, 1, 0, s>>>(iter, queues);
+    WOTK_LAUNCH_CHECK();
+    return WOTK_OK;
+}
+
+typedef int (*SweepFn)(const BatchArgs &, const BatchArgs &, int, int, int *, int *, hipStream_t);
+
+static SweepFn pick(int E) {
+    switch (E) {
+        case 1: return launch_sweeps<1>;
+        case 2: return launch_sweeps<2>;
+        case 3: return launch_sweeps<3>;
+        case 4: return launch_sweeps<4>;
+        case 6: return launch_sweeps<6>;
+        case 9: return launch_sweeps<9>;
+        case 12: return launch_sweeps<12>;
+        case 16: return launch_sweeps<16>;
+        default: return nullptr;
+    }
+}
+
+static int persistent_grid(int E) {
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
+    hipError_t e = hipSuccess;
+    switch (E) {
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<1>, kBThreads, 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<2>, kBThreads, 0); break;
+        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<3>, kBThreads, 0); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<4>, kBThreads, 0); break;
+        case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<6>, kBThreads, 0); break;
+        case 9: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<9>, kBThreads, 0); break;
+        case 12: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<12>, kBThreads, 0); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<16>, kBThreads, 0); break;
+    }
+    if (e != hipSuccess || per < 1) per = 1;
+    return ncu * per;
+}
+
+extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pairs, int32_t n_pairs, const double *G,
+                                   int32_t n_combos, const double *epsilon, const double *lambda1, double lambda2,
+                                   int64_t scaling_iter, double tau, double epsilon0, int64_t extra_iter,
+                                   int64_t inner_iter_max, double *f_out, double *g_out, int32_t graph_iters,
+                                   void *workspace, size_t workspace_bytes, wotk_stats *stats, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const int E = n_combos;
+    WOTK_REQUIRE(n_pairs > 0 && pairs && cost && G, "empty batch");
+    WOTK_REQUIRE(E >= 1 && E <= kMaxCombos && pick(E), "unsupported number of combinations %d", E);
+    WOTK_REQUIRE(scaling_iter >= 0 && extra_iter >= 0 && scaling_iter + extra_iter > 0 && inner_iter_max > 0,
+                 "bad iteration counts");
+    (void)tau;  // absorptions are exact in the log domain (see sinkhorn.hip)
+    for (int p = 0; p < n_pairs; ++p) {
+        const wotk_batch_pair &q = pairs[p];
+        WOTK_REQUIRE(q.n_i > 0 && q.n_j > 0 && q.n_i < (1 << 30) && q.n_j < (1 << 30), "pair %d: bad shape", p);
+        WOTK_REQUIRE(q.ldc % 4 == 0 && q.ldc >= round_up(q.n_j, 4) && q.ldct % 4 == 0 && q.ldct >= round_up(q.n_i, 4),
+                     "pair %d: leading dimensions must be multiples of 4 covering the padded rows", p);
+        WOTK_REQUIRE(q.c_offset % 4 == 0 && q.ct_offset % 4 == 0, "pair %d: offsets must be multiples of 4", p);
+    }
+    const int64_t n_iter = scaling_iter + extra_iter;
+    Arena ar(workspace, workspace_bytes);
+    BatchWork w;
+    int64_t nf, ng, ntf, ntg;
+    batch_carve(ar, pairs, n_pairs, E, n_iter, &nf, &ng, &w, &ntf, &ntg);
+    if (!workspace || !ar.ok) return fail(WOTK_EWORKSPACE, "batch workspace too small");
+
+    // host-side tables: pairs, tiles (longest rows first), iteration parameters
+    std::vector<PairDev> pf(n_pairs), pg(n_pairs);
+    std::vector<long long> goff(n_pairs), lwoff(n_pairs);
+    std::vector<int> nsrc(n_pairs);
+    std::vector<double> lognf(n_pairs), logng(n_pairs);
+    long long of = 0, og = 0, olw = 0;
+    for (int p = 0; p < n_pairs; ++p) {
+        const wotk_batch_pair &q = pairs[p];
+        pf[p] = PairDev{q.c_offset, q.ldc, (int)q.n_i, (int)q.n_j, of * E, og * E, olw, p};
+        pg[p] = PairDev{q.ct_offset, q.ldct, (int)q.n_j, (int)q.n_i, og * E, of * E, -1, p};
+        goff[p] = q.g_offset;
+        lwoff[p] = olw;
+        nsrc[p] = (int)q.n_i;
+        lognf[p] = std::log((double)q.n_j);
+        logng[p] = std::log((double)q.n_i);
+        of += round_up(q.n_i, 4);
+        og += round_up(q.n_j, 4);
+        olw += q.n_i;
+    }
+    auto tiles_for = [&](const std::vector<PairDev> &pd) {
+        std::vector<Tile> t;
+        for (int p = 0; p < n_pairs; ++p)
+            for (int r = 0; r < pd[p].nrows; r += kBRows) t.push_back(Tile{p, r});
+        std::stable_sort(t.begin(), t.end(), [&](const Tile &a, const Tile &b) {
+            return pd[a.side_pair].ncols > pd[b.side_pair].ncols;
+        });
+        return t;
+    };
+    std::vector<Tile> tf = tiles_for(pf), tg = tiles_for(pg);
+    std::vector<IterParam> table((size_t)(n_iter + 1) * E);
+    for (int e = 0; e < E; ++e) {
+        auto reg = [&](int64_t n) { return (epsilon0 - epsilon[e]) * std::exp(-(double)n) + epsilon[e]; };
+        for (int64_t n = 0; n <= n_iter; ++n) {
+            const double eps = n < scaling_iter ? reg(n / inner_iter_max) : reg(scaling_iter / inner_iter_max);
+            table[(size_t)n * E + e] = IterParam{kLog2E / eps, eps, lambda1[e] / (lambda1[e] + eps), lambda2 / (lambda2 + eps)};
+        }
+    }
+    WOTK_HIP(hipMemcpyAsync(w.pairs_f, pf.data(), sizeof(PairDev) * n_pairs, hipMemcpyHostToDevice, s));
+    WOTK_HIP(hipMemcpyAsync(w.pairs_g, pg.data(), sizeof(PairDev) * n_pairs, hipMemcpyHostToDevice, s));
+    WOTK_HIP(hipMemcpyAsync(w.tiles_f, tf.data(), sizeof(Tile) * tf.size(), hipMemcpyHostToDevice, s));
+    WOTK_HIP(hipMemcpyAsync(w.tiles_g, tg.data(), sizeof(Tile) * tg.size(), hipMemcpyHostToDevice, s));
+    WOTK_HIP(hipMemcpyAsync(w.table, table.data(), sizeof(IterParam) * table.size(), hipMemcpyHostToDevice, s));
+    WOTK_HIP(hipMemcpyAsync(w.g_off, goff.data(), sizeof(long long) * n_pairs, hipMemcpyHostToDevice, s));
+    WOTK_HIP(hipMemcpyAsync(w.lw_off, lwoff.data(), sizeof(long long) * n_pairs, hipMemcpyHostToDevice, s));
+    WOTK_HIP(hipMemcpyAsync(w.n_src, nsrc.data(), sizeof(int) * n_pairs, hipMemcpyHostToDevice, s));
+    WOTK_HIP(hipMemcpyAsync(w.lognf, lognf.data(), sizeof(double) * n_pairs, hipMemcpyHostToDevice, s));
+    WOTK_HIP(hipMemcpyAsync(w.logng, logng.data(), sizeof(double) * n_pairs, hipMemcpyHostToDevice, s));
+
+    // potentials start at 0 (u = v = 0, a = b = 1), padding columns at -inf
+    batch_fill_kernel<<<1024, 256, 0, s>>>(w.f, nf * E, 0.0);
+    batch_fill_kernel<<<1024, 256, 0, s>>>(w.g, ng * E, 0.0);
+    WOTK_LAUNCH_CHECK();
+    {
+        // -inf on the padding rows of each side (read as "other" potentials)
+        std::vector<double> ninf(4 * E, -INFINITY);
+        long long a = 0, b = 0;
+        for (int p = 0; p < n_pairs; ++p) {
+            const int64_t pi = round_up(pairs[p].n_i, 4) - pairs[p].n_i, pj = round_up(pairs[p].n_j, 4) - pairs[p].n_j;
+            if (pi) WOTK_HIP(hipMemcpyAsync(w.f + (a + pairs[p].n_i) * E, ninf.data(), sizeof(double) * pi * E, hipMemcpyHostToDevice, s));
+            if (pj) WOTK_HIP(hipMemcpyAsync(w.g + (b + pairs[p].n_j) * E, ninf.data(), sizeof(double) * pj * E, hipMemcpyHostToDevice, s));
+            a += round_up(pairs[p].n_i, 4);
+            b += round_up(pairs[p].n_j, 4);
+        }
+    }
+    batch_growth_kernel<<<n_pairs, 256, 0, s>>>(G, w.g_off, w.n_src, w.logw, w.lw_off, w.logq);
+    batch_reset_kernel<<<1, 1, 0, s>>>(w.iter, w.queues, 0);
+    WOTK_LAUNCH_CHECK();
+    WOTK_HIP(hipStreamSynchronize(s));  // host tables above are released at return
+
+    BatchArgs af{cost, w.pairs_f, w.tiles_f, (int)ntf, w.table, w.iter, E, 0, w.f, w.g, w.logw, w.logq, w.lognf, w.queues};
+    BatchArgs ag{cost, w.pairs_g, w.tiles_g, (int)ntg, w.table, w.iter, E, 1, w.g, w.f, w.logw, w.logq, w.logng, w.queues + 1};
+    const int grid = persistent_grid(E);
+    const int grid_f = (int)std::min<int64_t>(grid, ntf), grid_g = (int)std::min<int64_t>(grid, ntg);
+    SweepFn sweep = pick(E);
+
+    hipEvent_t ev[2];
+    WOTK_HIP(hipEventCreate(&ev[0]));
+    WOTK_HIP(hipEventCreate(&ev[1]));
+    int64_t done = 0;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipStream_t cs = nullptr;
+    if (graph_iters > 0 && n_iter >= graph_iters) {
+        // capture graph_iters iterations once on a private stream, replay them
+        WOTK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        WOTK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        for (int t = 0; t < graph_iters; ++t) {
+            int rc = sweep(af, ag, grid_f, grid_g, w.iter, w.queues, cs);
+            if (rc) {
+                hipStreamEndCapture(cs, &graph);
+                return rc;
+            }
+        }
+        WOTK_HIP(hipStreamEndCapture(cs, &graph));
+        WOTK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    WOTK_HIP(hipEventRecord(ev[0], s));
+    if (exec) {
+        while (n_iter - done >= graph_iters) {
+            WOTK_HIP(hipGraphLaunch(exec, s));
+            done += graph_iters;
+        }
+    }
+    for (; done < n_iter; ++done) {
+        int rc = sweep(af, ag, grid_f, grid_g, w.iter, w.queues, s);
+        if (rc) return rc;
+    }
+    WOTK_HIP(hipEventRecord(ev[1], s));
+    // outputs: [pair][combo][n] contiguous blocks
+    {
+        long long a = 0, b = 0, oa = 0, ob = 0;
+        for (int p = 0; p < n_pairs; ++p) {
+            for (int e = 0; e < E; ++e) {
+                if (f_out) batch_gather_kernel<<<64, 256, 0, s>>>(w.f + a * E, (int)pairs[p].n_i, E, e, f_out + oa + (long long)e * pairs[p].n_i);
+                if (g_out) batch_gather_kernel<<<64, 256, 0, s>>>(w.g + b * E, (int)pairs[p].n_j, E, e, g_out + ob + (long long)e * pairs[p].n_j);
+            }
+            a += round_up(pairs[p].n_i, 4);
+            b += round_up(pairs[p].n_j, 4);
+            oa += (long long)E * pairs[p].n_i;
+            ob += (long long)E * pairs[p].n_j;
+        }
+        WOTK_LAUNCH_CHECK();
+    }
+    WOTK_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    WOTK_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    if (cs) hipStreamDestroy(cs);
+    if (stats) {
+        *stats = wotk_stats{};
+        stats->iterations = n_iter;
+        stats->sweeps = 2 * n_iter;
+        stats->sweep_ms = ms;
+        stats->epsilon_final = table[(size_t)n_iter * E].eps;
+    }
+    return WOTK_OK;
+}

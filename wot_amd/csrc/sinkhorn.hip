@@ -749,6 +749,16 @@ extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const flo
     return WOTK_OK;
 }
 
+extern "C" int wotk_plan(const float *C, int64_t ldc, int64_t I, int64_t J, const double *f, const double *g,
+                         double epsilon, double divisor, double *plan, int64_t ldp, double *row_sums,
+                         void *stream) {
+    WOTK_REQUIRE(I > 0 && J > 0 && I < (1ll << 31) && J < (1ll << 31), "bad shape");
+    WOTK_REQUIRE(ldc % 4 == 0 && ldc >= round_up(J, 4), "ldc must be a multiple of 4 >= J");
+    WOTK_REQUIRE(!plan || ldp >= J, "ldp < J");
+    return launch_plan(C, ldc, (int)I, (int)J, f, g, epsilon, divisor, plan, ldp, row_sums, nullptr, 0,
+                       (hipStream_t)stream);
+}
+
 extern "C" int wotk_primal_dual(const float *C, int64_t ldc, const float *CT, int64_t ldct,
                                 int64_t I, int64_t J, const double *G, const double *f,
                                 const double *g, double lambda1, double lambda2, double epsilon,

@@ -43,15 +43,18 @@ def day_pair(n_source=2000, n_target=2500, dim=30, n_components=5, mean_scale=3.
 
 
 def time_course(n_days=39, min_cells=1000, max_cells=20000, dim=30, n_components=20,
-                mean_scale=3.0, drift=0.25, sigma=1.0, growth_sigma=0.3, seed=0):
+                mean_scale=3.0, drift=0.25, sigma=1.0, growth_sigma=0.3, seed=0, counts=None):
     """A time course of ``n_days`` days (``n_days - 1`` consecutive pairs).
-    Cells per day ~ U{min_cells..max_cells}; component means drift linearly
+    Cells per day ~ U{min_cells..max_cells} (or the given ``counts``);
+    component means drift linearly
     (mu_k(t) = mu_k + t * drift * w_k, w_k ~ N(0, I)).  Returns (coords per
     day, growth per day, cell counts)."""
     rng = np.random.default_rng(seed)
     mu0 = mean_scale * rng.standard_normal((n_components, dim))
     w = rng.standard_normal((n_components, dim))
-    counts = rng.integers(min_cells, max_cells + 1, size=n_days)
+    drawn = rng.integers(min_cells, max_cells + 1, size=n_days)
+    counts = np.asarray(counts if counts is not None else drawn)
+    n_days = len(counts)
     coords, growth = [], []
     for t in range(n_days):
         mu_t = mu0 + t * drift * w
