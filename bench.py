@@ -121,17 +121,368 @@ def cpu_sample(x, y, growth):
     return per_map, threads, dict(t_cost_s=t_cost, t_iter_s=t_it, t_build_s=t_build, sample_wall_s=t_cost + t_a + t_b)
 
 
+TC = dict(n_days=39, min_cells=1000, max_cells=20000, dim=30, n_components=20, seed=2)
+GRID = dict(epsilons=(0.01, 0.05, 0.1), lambda1s=(0.5, 1.0, 5.0))
+
+
+def bench_timecourse(args):
+    """38 consecutive day pairs (cells per day ~ U[1000, 20000], 30-d, 20
+    drifting components) x the grid eps {0.01, 0.05, 0.1} x lambda1 {0.5, 1, 5}
+    = 342 fixed_iters solves in one batched run (hipGraph replay of 50-iteration
+    blocks).  value = solves (transport maps) per second."""
+    import torch
+    import wot_amd.ot as ot
+    from wot_amd.ot import batch
+    from wot_amd.synthetic import time_course
+    coords, growth, counts = time_course(**TC)
+    sched = dict(SOLVER)
+    sched.pop('growth_iters')
+    for k in ('epsilon', 'lambda1'):
+        sched.pop(k)
+    if args.iters is not None:
+        sched['extra_iter'] = sched['extra_iter'] * args.iters // sched['scaling_iter']
+        sched['scaling_iter'] = args.iters
+    n_iter = sched['scaling_iter'] + sched['extra_iter']
+    pair_ij = [int(counts[t]) * int(counts[t + 1]) for t in range(len(counts) - 1)]
+    sum_ij = float(sum(pair_ij))
+    n_combo = len(GRID['epsilons']) * len(GRID['lambda1s'])
+    if args.warmup:
+        log('warmup (2 iterations)')
+        batch.solve_time_course(coords[:3], growth[:3], scaling_iter=2, extra_iter=0, **GRID)
+    torch.cuda.synchronize()
+    log('time course: %d pairs, %d combinations, sum I*J = %.3e, %d iterations' %
+        (len(pair_ij), n_combo, sum_ij, n_iter))
+    t0 = time.perf_counter()
+    res = batch.solve_time_course(coords, growth, graph_iters=50, **GRID, **sched)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n_solves = len(pair_ij) * n_combo
+    sweep_s = res.stats['sweep_ms'] / 1000.0
+    avg_sweep = sweep_s / (2 * n_iter)
+    # algorithmic work of one sweep over the whole batch: 4 bytes per cost entry
+    # (one read shared by all combinations) and one exp per (entry, combination)
+    alg_bytes = 4.0 * sum_ij
+    out = {
+        'metric': 'transport maps/sec and Sinkhorn iters/sec at \u22641e-5 rel err, fused vs plain GPU',
+        'value': n_solves / elapsed, 'unit': 'maps/s', 'n_gpus': 1, 'steps': 1, 'warmup': int(bool(args.warmup)),
+        'ms_per_step': 1000.0 * elapsed, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64/f32', 'data': 'synthetic (seeded drifting Gaussian mixture, 39 days)',
+        'config': {'workload': '38-pair time course x eps{0.01,0.05,0.1} x lambda1{0.5,1,5}, fixed_iters',
+                   'cells_total': int(sum(counts)), 'sum_IJ': sum_ij, 'solves': n_solves, **sched},
+        'sinkhorn_iters_per_s': n_solves * n_iter / elapsed,
+        'sweep_loop_s': sweep_s,
+        'roofline': {'bound': 'valu/exp', 'kernel': 'batch_sweep_kernel<9>', 'avg_launch_us': avg_sweep * 1e6,
+                     'hbm_achieved': alg_bytes / avg_sweep / 1e9, 'hbm_peak': HBM_PEAK_GBS,
+                     'exp_per_s': n_combo * sum_ij / avg_sweep, 'unit': 'GB/s',
+                     'achieved': alg_bytes / avg_sweep / 1e9, 'peak': HBM_PEAK_GBS,
+                     'frac': alg_bytes / avg_sweep / 1e9 / HBM_PEAK_GBS, 'traffic': None},
+    }
+    # parity: a few (pair, combination) maps against the single-pair solver
+    # (itself checked against the FP64 oracle) on the same device cost
+    worst = 0.0
+    for p in (0, int(np.argmax(pair_ij)), len(pair_ij) - 1):
+        cost = ot.DeviceCost.from_coordinates(coords[p], coords[p + 1])
+        for e in (0, 4, 8):
+            eps, lam1 = res.combos[e]
+            R1 = ot.transport_stablev2(cost, G=growth[p], device_out=True, epsilon=eps, lambda1=lam1, **sched)
+            R2 = res.plan(p, e, device_out=True)
+            worst = max(worst, float(torch.linalg.norm(R1 - R2) / torch.linalg.norm(R1)))
+            del R1, R2
+    out['parity_vs_single_solver_rel_frob'] = worst
+    if not args.no_baselines:
+        out.update(_timecourse_baselines(torch, coords, growth, pair_ij, n_combo, sched, n_iter, elapsed))
+    return out
+
+
+def _fit_and_sum(samples, pair_ij, n_combo):
+    """least-squares t = a + b * I*J over timed samples, summed over all solves"""
+    x = np.array([s[0] for s in samples], dtype=np.float64)
+    t = np.array([s[1] for s in samples], dtype=np.float64)
+    A = np.vstack([np.ones_like(x), x]).T
+    a, b = np.linalg.lstsq(A, t, rcond=None)[0]
+    a, b = max(a, 0.0), max(b, 0.0)
+    return n_combo * sum(a + b * ij for ij in pair_ij), a, b
+
+
+def _timecourse_baselines(torch, coords, growth, pair_ij, n_combo, sched, n_iter, elapsed):
+    from wot_amd import plain_gpu
+    from wot_amd.ot import device as dev
+    order = np.argsort(pair_ij)
+    picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    samples = []
+    for p in picks:
+        X = torch.from_numpy(coords[p]).cuda()
+        Y = torch.from_numpy(coords[p + 1]).cuda()
+        c64 = dev.sqeuclid_f64(X, Y)
+        v = torch.sort(c64.view(-1)).values
+        n = v.numel()
+        med = (v[n // 2 - 1] + v[n // 2]) / 2 if n % 2 == 0 else v[n // 2]
+        C = c64 / med
+        del v, c64
+        plain_gpu.transport_stablev2(C, G=growth[p], epsilon=0.05, lambda1=1.0,
+                                     **dict(sched, scaling_iter=3, extra_iter=0))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain_gpu.transport_stablev2(C, G=growth[p], epsilon=0.05, lambda1=1.0, **sched)
+        torch.cuda.synchronize()
+        samples.append((pair_ij[p], time.perf_counter() - t0))
+        del C
+        torch.cuda.empty_cache()
+        log('plain GPU sample pair %d (I*J=%d): %.2f s' % (p, pair_ij[p], samples[-1][1]))
+    plain_total, a, b = _fit_and_sum(samples, pair_ij, n_combo)
+    # CPU: the FP64 oracle on the smallest pair, 50 + 50 iterations, scaled by I*J
+    from oracle import ot_oracle as O
+    p = picks[0]
+    C = O.default_cost_matrix(coords[p], coords[p + 1])
+    t0 = time.perf_counter()
+    O.transport_stablev2(C, 1.0, SOLVER['lambda2'], 0.05, 50, growth[p], 1e4, 1.0, 0, 50)
+    ta = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.transport_stablev2(C, 1.0, SOLVER['lambda2'], 0.05, 50, growth[p], 1e4, 1.0, 50, 50)
+    tb = time.perf_counter() - t0
+    t_it = max((tb - ta) / 50.0, 0.0)
+    t_build = max((ta - 50 * t_it) / 2.0, 0.0)
+    n_build = 1 + sched['scaling_iter'] // sched['inner_iter_max']
+    per_ij = (n_build * t_build + n_iter * t_it) / pair_ij[p]
+    cpu_total = n_combo * per_ij * sum(pair_ij)
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get('num_threads', 1) for i in threadpool_info()] + [1])
+    except Exception:  # pragma: no cover
+        threads = os.cpu_count()
+    n_solves = len(pair_ij) * n_combo
+    return {
+        'plain_gpu': {'value': n_solves / plain_total, 'unit': 'maps/s',
+                      'sample': 'torch FP64 dense-K restatement: full solves of the smallest, median and largest '
+                                'pairs timed, t = a + b*I*J fitted and summed over all 342 solves',
+                      'fit_a_s': a, 'fit_b_s': b, 'samples': samples},
+        'speedup_vs_plain_gpu': plain_total / elapsed,
+        'cpu_baseline': {'value': n_solves / cpu_total, 'unit': 'maps/s', 'cores': threads, 'kind': 'port',
+                         'sample': 'FP64 numpy oracle on the smallest pair (50 scheduled + 50 extra iterations '
+                                   'timed apart), scaled by I*J to every solve'},
+        'speedup_vs_cpu': cpu_total / elapsed,
+    }
+
+
+def bench_adjoint(args):
+    """Gradient of the converged objective w.r.t. 8000 x 30 FP32 source
+    embeddings and 8000 FP32 log growth rates against 8000 target cells:
+    forward (cost, converged duality-gap solve, FP64 primal) + adjoint kernel.
+    Checks: FP64 autograd of the primal at the same plan, and central finite
+    differences (h = 1e-4) of the FP64-converged objective on 64 random
+    coordinates of a 300 x 360 sub-problem (FD at 8000 x 8000 would need the
+    objective to ~1e-13, beyond an FP32 cost stream)."""
+    import torch
+    import wot_amd.ot as ot
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(8000, 8000, dim=30, n_components=10, seed=5)
+    x = torch.from_numpy(dp.x).float().cuda().requires_grad_(True)
+    lg = torch.from_numpy(np.log(dp.growth)).float().cuda().requires_grad_(True)
+    y = torch.from_numpy(dp.y).float().cuda()
+    if args.warmup:
+        xs = x[:500].detach().requires_grad_(True)
+        ot.ot_objective(xs, lg[:500].detach(), y[:600]).backward()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(max(args.steps, 1)):
+        x.grad = None
+        lg.grad = None
+        t0 = time.perf_counter()
+        val = ot.ot_objective(x, lg, y)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        val.backward()
+        torch.cuda.synchronize()
+        times.append((t1 - t0, time.perf_counter() - t1))
+    fwd = float(np.mean([t[0] for t in times]))
+    bwd = float(np.mean([t[1] for t in times]))
+    out = {'metric': 'gradient evaluations/sec of the converged OT objective (forward solve + adjoint)',
+           'value': 1.0 / (fwd + bwd), 'unit': 'grads/s', 'n_gpus': 1, 'steps': len(times),
+           'warmup': int(bool(args.warmup)), 'ms_per_step': 1000.0 * (fwd + bwd), 'higher_is_better': True,
+           'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32 in / f64 adjoint',
+           'data': 'synthetic (seeded Gaussian mixture, 30-d)',
+           'config': {'workload': 'd OT / d(x, log g), 8000 x 8000, d=30, duality_gap solver'},
+           'forward_s': fwd, 'adjoint_s': bwd, 'objective': float(val)}
+    out.update(_adjoint_checks(torch, ot, dp, x, lg, y))
+    return out
+
+
+def _adjoint_checks(torch, ot, dp, x, lg, y):
+    from wot_amd.ot import adjoint as adj
+    from wot_amd.ot import device as dev
+    from oracle import ot_oracle as O
+    # FP64 autograd of the primal at the same converged plan (envelope theorem)
+    X = x.detach().double()
+    Y = y.double()
+    G = torch.exp(lg.detach().double())
+    cost = ot.DeviceCost.from_coordinates(X, Y)
+    prm = dict(adj.DEFAULTS)
+    res = dev.solve(cost, G, 'duality_gap', prm, want_plan=False, want_potentials=True)
+    eps = res['stats']['epsilon_final']
+    gx, glg = adj.adjoint(cost, X, Y, G, res['f'], res['g'], prm['lambda1'], prm['lambda2'], eps)
+    xx = X.clone().requires_grad_(True)
+    ll = lg.detach().double().clone().requires_grad_(True)
+    D = (xx * xx).sum(1)[:, None] + (Y * Y).sum(1)[None, :] - 2.0 * xx @ Y.T
+    Dd = dev.sqeuclid_f64(X, Y)
+    D = D + (Dd - D).detach()
+    mid = cost.median_index.cpu().numpy()
+    flat = D.reshape(-1)
+    m = 0.5 * (flat[int(mid[0])] + flat[int(mid[1])])
+    C = D / m
+    n_i, n_j = C.shape
+    p = torch.exp(ll)
+    q = p.mean()
+    with torch.no_grad():
+        R = torch.exp((res['f'][:, None] + res['g'][None, :] - C.detach()) / eps)
+    K = torch.exp(-C / eps)
+    xr, yc = R.sum(1) / n_j, R.sum(0) / n_i
+    P = (prm['lambda1'] * (1.0 / n_i * (xr * torch.log(xr / p) - xr + p)).sum()
+         + prm['lambda2'] * (1.0 / n_j * (yc * torch.log(yc / q) - yc + q)).sum()
+         + (eps * (R * torch.log(R) - R + K).sum() + (R * C).sum()) / (n_i * n_j))
+    P.backward()
+    e_x = float(torch.linalg.norm(gx - xx.grad) / torch.linalg.norm(xx.grad))
+    e_g = float(torch.linalg.norm(glg - ll.grad) / torch.linalg.norm(ll.grad))
+    del D, Dd, C, R, K, P
+    torch.cuda.empty_cache()
+    # central finite differences of the FP64-converged objective (oracle) on a sub-problem
+    sub_i, sub_j = 300, 360
+    xs, ys, lgs = dp.x[:sub_i], dp.y[:sub_j], np.log(dp.growth[:sub_i])
+    fd_prm = dict(lambda1=1.0, lambda2=50.0, epsilon=0.05, scaling_iter=2000, tau=1e4, epsilon0=0.05,
+                  extra_iter=0, inner_iter_max=50)
+    xt = torch.from_numpy(xs).cuda().requires_grad_(True)
+    lt = torch.from_numpy(lgs).cuda().requires_grad_(True)
+    ot.ot_objective(xt, lt, torch.from_numpy(ys).cuda(), solver='fixed_iters', **fd_prm).backward()
+    gpu_gx = xt.grad.cpu().numpy()
+    rng = np.random.default_rng(0)
+    coords = [(int(rng.integers(sub_i)), int(rng.integers(xs.shape[1]))) for _ in range(64)]
+    h = 1e-4
+    fd, an = [], []
+    for (i, k) in coords:
+        a, b = xs.copy(), xs.copy()
+        a[i, k] += h
+        b[i, k] -= h
+        fa = O.objective_and_grad(a, lgs, ys, **fd_prm)[0]
+        fb = O.objective_and_grad(b, lgs, ys, **fd_prm)[0]
+        fd.append((fa - fb) / (2 * h))
+        an.append(gpu_gx[i, k])
+    fd, an = np.array(fd), np.array(an)
+    e_fd = float(np.linalg.norm(fd - an) / np.linalg.norm(fd))
+    return {'adjoint_vs_fp64_autograd_rel': {'grad_x': e_x, 'grad_log_g': e_g},
+            'adjoint_vs_central_fd_rel_64coords': e_fd,
+            'fd_setup': '300 x 360 sub-problem, eps0 = eps = 0.05, 2000 iterations, h = 1e-4, FP64 oracle'}
+
+
+def bench_large(args):
+    """One 60,000 x 60,000 pair (50-d, 30 components) solved without a stored
+    cost (cost + exp + LSE fused per tile on the matrix cores), 3-term and
+    6-term split-BF16 cross terms, full fixed_iters schedule; then a 5-map
+    long-range coupling chain (FP32 MFMA GEMMs) with its row-normalised error
+    against the FP64 chain."""
+    import torch
+    import wot_amd.ot as ot
+    import wot_amd.tmap as tm
+    from wot_amd.ot import device as dev
+    from wot_amd.ot import large
+    from wot_amd.synthetic import day_pair, time_course
+    n = args.large_n
+    dp = day_pair(n, n, dim=50, n_components=30, seed=6)
+    sched = dict(SOLVER)
+    sched.pop('growth_iters')
+    if args.iters is not None:
+        sched['extra_iter'] = sched['extra_iter'] * args.iters // sched['scaling_iter']
+        sched['scaling_iter'] = args.iters
+    n_iter = sched['scaling_iter'] + sched['extra_iter']
+    out = {'metric': 'transport maps/sec and Sinkhorn iters/sec at \u22641e-5 rel err, fused vs plain GPU',
+           'unit': 'maps/s', 'n_gpus': 1, 'steps': 1, 'warmup': 0, 'higher_is_better': True, 'scaling': 'weak',
+           'vs_baseline': None, 'dtype': 'bf16 split / f32 acc / f64 potentials',
+           'data': 'synthetic (seeded 30-component Gaussian mixture, 50-d)',
+           'config': {'workload': '%d x %d pair, cost never stored, fixed_iters' % (n, n), **sched}}
+    X = torch.from_numpy(dp.x).cuda()
+    Y = torch.from_numpy(dp.y).cuda()
+    G = torch.from_numpy(dp.growth).cuda()
+    res = {}
+    for terms in (3, 6):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = large.transport_stablev2_coords(X, Y, G, terms=terms, **sched)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[terms] = r
+        sweep = r.stats['sweep_ms'] / 1000.0 / (2 * n_iter)
+        out['terms%d' % terms] = {'seconds': el, 'maps_per_s': 1.0 / el, 'sinkhorn_iters_per_s': n_iter / el,
+                                  'avg_sweep_us': sweep * 1e6, 'entries_per_s': n * n / sweep,
+                                  'mfma_tflops': 2.0 * terms * n * n * (((50 + 15) // 16) * 16) / sweep / 1e12}
+        log('terms %d: %.2f s' % (terms, el))
+    out['value'] = out['terms3']['maps_per_s']
+    out['ms_per_step'] = 1000.0 * out['terms3']['seconds']
+    out['sinkhorn_iters_per_s'] = out['terms3']['sinkhorn_iters_per_s']
+    # full-size cross-check on a short schedule against the stored-cost solver
+    short = dict(sched, scaling_iter=100, extra_iter=0)
+    cost = ot.DeviceCost.from_coordinates(X, Y)
+    ref = dev.solve(cost, G, 'fixed_iters', short, want_plan=False, want_potentials=True)
+    rows = torch.arange(0, n, max(n // 512, 1), device='cuda')
+    D = dev.sqeuclid_f64(X[rows], Y)
+    C = D / cost.median
+    eps_s = ref['stats']['epsilon_final']
+    Rref = torch.exp((ref['f'][rows][:, None] + ref['g'][None, :] - C) / eps_s)
+    for terms in (3, 6):
+        r = large.transport_stablev2_coords(X, Y, G, terms=terms, **short)
+        R = torch.exp((r.f[rows][:, None] + r.g[None, :] - C) / eps_s)
+        out['terms%d' % terms]['rel_frob_vs_stored_cost_100it_512rows'] = float(torch.linalg.norm(R - Rref) /
+                                                                                torch.linalg.norm(Rref))
+    del cost, D, C, R, Rref
+    torch.cuda.empty_cache()
+    # 5-map chain: 6 days x 12,000 cells solved with the stored-cost solver
+    coords, growth, counts = time_course(dim=50, n_components=30, seed=7, counts=[12000] * 6)
+    maps = []
+    for t in range(5):
+        cst = ot.DeviceCost.from_coordinates(coords[t], coords[t + 1])
+        maps.append(ot.transport_stablev2(cst, G=growth[t], device_out=True, **sched))
+        del cst
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    chain32 = tm.chain_product([m.float() for m in maps])
+    torch.cuda.synchronize()
+    t_chain = time.perf_counter() - t0
+    ref64 = maps[0]
+    for m in maps[1:]:
+        ref64 = ref64 @ m
+    rn = chain32.double() / chain32.double().sum(1, keepdim=True)
+    rr = ref64 / ref64.sum(1, keepdim=True)
+    out['chain'] = {'maps': 5, 'cells_per_day': 12000, 'seconds': t_chain,
+                    'tflops': 4 * 2.0 * 12000 ** 3 / t_chain / 1e12,
+                    'rel_frob_vs_fp64': float(torch.linalg.norm(chain32.double() - ref64) / torch.linalg.norm(ref64)),
+                    'row_normalised_max_abs_err': float((rn - rr).abs().max()),
+                    'row_normalised_rel_frob': float(torch.linalg.norm(rn - rr) / torch.linalg.norm(rr))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--no-baselines', action='store_true', help='skip the plain-GPU and CPU samples')
+    ap.add_argument('--large-n', type=int, default=60000, help='large only: cells per day')
+    ap.add_argument('--config', default='pair', choices=['pair', 'timecourse', 'adjoint', 'large'],
+                    help='pair: one 15k x 15k map (default); timecourse: 38 pairs x 9-point grid; '
+                         'adjoint: gradient of the converged objective, 8k x 8k')
+    ap.add_argument('--iters', type=int, default=None,
+                    help='timecourse / large: override scaling_iter (extra_iter scaled alike) for quick probes')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (got --gpus %d)' % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.config == 'timecourse':
+        print(json.dumps(bench_timecourse(args)), flush=True)
+        return
+    if args.config == 'adjoint':
+        print(json.dumps(bench_adjoint(args)), flush=True)
+        return
+    if args.config == 'large':
+        print(json.dumps(bench_large(args)), flush=True)
+        return
 
     import torch
     import wot_amd.ot as ot

@@ -162,6 +162,23 @@ int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pairs, int32_t
                         double *g_out, int32_t graph_iters, void *workspace,
                         size_t workspace_bytes, wotk_stats *stats, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * transport_stablev2 (optimal_transport.py:167-236) on the cost of
+ * ot_model.py:242-253 computed from the coordinates X (I x d), Y (J x d) and
+ * never stored: every sweep regenerates C from the coordinates on the matrix
+ * cores (split-BF16 cross term, `terms` = 3 or 6 products, FP32 accumulate)
+ * with FP64 potentials; the median is the exact median of the FP64 distances.
+ * d <= 128.  f_out (I), g_out (J), median_out (1): device, may be NULL.
+ * ------------------------------------------------------------------------- */
+size_t wotk_fused_workspace_size(int64_t I, int64_t J, int64_t d);
+int wotk_transport_stablev2_coords(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
+                                   const double *G, double lambda1, double lambda2, double epsilon,
+                                   int64_t scaling_iter, double tau, double epsilon0,
+                                   int64_t extra_iter, int64_t inner_iter_max, int32_t terms,
+                                   double *f_out, double *g_out, double *median_out,
+                                   void *workspace, size_t workspace_bytes, wotk_stats *stats,
+                                   void *stream);
+
 /* Primal / dual objectives of optimal_transport.py:45-62 at the plan
  * R = exp((f_i + g_j - C_ij)/eps) (un-normalised), FP64 reductions.
  * out[0] = primal, out[1] = dual (device or host pointer to 2 doubles: host). */

@@ -1,0 +1,64 @@
+"""Never-materialised solver (cost regenerated per sweep on MFMA) against the
+stored-cost solver (FP32 cost, FP64 exponents; itself checked against the
+FP64 oracle) on the same inputs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SCHED = dict(lambda1=1.0, lambda2=50.0, epsilon=0.05, scaling_iter=300, tau=1e4, epsilon0=1.0, extra_iter=100,
+             inner_iter_max=50)
+
+
+@pytest.fixture(scope='module')
+def mods():
+    import torch
+    assert torch.cuda.is_available()
+    import wot_amd.ot as ot
+    from wot_amd.ot import device as dev
+    from wot_amd.ot import large
+    return torch, ot, dev, large
+
+
+def _compare(torch, ot, dev, large, x, y, G, terms, sched):
+    res = large.transport_stablev2_coords(x, y, G, terms=terms, **sched)
+    cost = ot.DeviceCost.from_coordinates(x, y)
+    ref = dev.solve(cost, G, 'fixed_iters', sched, want_plan=True, want_potentials=True)
+    assert float(res.median.cpu()[0]) == float(cost.median.cpu()[0])
+    rows = torch.arange(x.shape[0], device='cuda')
+    R = res.plan_rows(x, y, rows)
+    Rref = ref['plan']
+    err = float(torch.linalg.norm(R - Rref) / torch.linalg.norm(Rref))
+    return err, res
+
+
+@pytest.mark.parametrize('terms,tol', [(6, 1e-5), (3, 1e-3)])
+def test_fused_matches_stored_cost(mods, terms, tol):
+    torch, ot, dev, large = mods
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(700, 900, dim=50, n_components=8, seed=41)
+    err, res = _compare(torch, ot, dev, large, dp.x, dp.y, dp.growth, terms, SCHED)
+    print('fused terms=%d rel frob vs stored-cost solver: %.3e' % (terms, err))
+    assert err < tol
+
+
+def test_fused_repair_rows(mods):
+    """eps0 = eps = 0.01 with an outlier source cell: its fixed-shift row sum
+    underflows in the first sweep and the repair kernel must recompute it."""
+    torch, ot, dev, large = mods
+    from wot_amd.synthetic import time_course
+    coords, growth, _ = time_course(dim=10, n_components=4, seed=3, counts=[120, 140])
+    x = coords[0].copy()
+    x[0] += 8.0
+    sched = dict(SCHED, epsilon=0.01, epsilon0=0.01)
+    err, res = _compare(torch, ot, dev, large, x, coords[1], growth[0], 6, sched)
+    assert np.isfinite(res.f.cpu().numpy()).all()
+    assert err < 1e-5, err
+
+
+def test_fused_odd_sizes_and_dims(mods):
+    torch, ot, dev, large = mods
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(97, 131, dim=7, n_components=3, seed=5)
+    err, _ = _compare(torch, ot, dev, large, dp.x, dp.y, dp.growth, 6, SCHED)
+    assert err < 1e-5, err

@@ -1,0 +1,531 @@
+// Fixed-iteration solve (transport_stablev2 semantics,
+// wot/ot/optimal_transport.py:167-236) on the cost of ot_model.py:242-253
+// without ever materialising the cost or the kernel: every sweep recomputes
+// C_ij from the coordinates tile by tile on the matrix cores.
+//
+// Coordinates are centred on the mean of both days and scaled by
+// 1/sqrt(median), so C = |x'|^2 + |y'|^2 - 2 x'.y' directly.  The cross term
+// x'.y' is a split-BF16 MFMA product (x' = hi + mid + lo):
+//   3 terms: hi.hi + hi.lo' + lo'.hi  (lo' = bf16(x' - hi); ~2^-16 relative)
+//   6 terms: hi.hi + hi.mid + mid.hi + hi.lo + lo.hi + mid.mid (~2^-24 relative)
+// accumulated in FP32 (v_mfma_f32_32x32x16_bf16).  Per entry the exponent is
+// formed in FP64 against the row's previous potential (a fixed shift, as in
+// the batched sweep); rows whose sum under- or overflows are queued and
+// recomputed by a repair kernel from FP64 direct distances with an online
+// maximum.  The median of the exact FP64 distances (the reference's
+// normalisation) comes from a radix select that regenerates the distances in
+// every pass.
+#include "common.h"
+
+namespace wotk {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kFThreads = 256;
+
+// ---------------------------------------------------------------------------
+// preparation
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void fz_mean_kernel(const double *__restrict__ X, int n_i, const double *__restrict__ Y,
+                                                      int n_j, int d, double *__restrict__ mean) {
+    __shared__ double red[4];
+    const int k = blockIdx.x;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n_i; i += blockDim.x) acc += X[(int64_t)i * d + k];
+    for (int j = threadIdx.x; j < n_j; j += blockDim.x) acc += Y[(int64_t)j * d + k];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) mean[k] = (red[0] + red[1] + red[2] + red[3]) / (n_i + n_j);
+}
+
+// x' = (x - mean) / sqrt(median): FP64 copy, FP64 norm, bf16 split into
+// nterm pieces (hi, mid, lo) with K padded to kp (pitch kp)
+__global__ __launch_bounds__(256) void fz_split_kernel(const double *__restrict__ X, int n, int d, int kp,
+                                                       const double *__restrict__ mean,
+                                                       const double *__restrict__ median, int npiece,
+                                                       __bf16 *__restrict__ p0, __bf16 *__restrict__ p1,
+                                                       __bf16 *__restrict__ p2, double *__restrict__ xs,
+                                                       double *__restrict__ norm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double sc = 1.0 / sqrt(median[0]);
+    double acc = 0.0;
+    for (int k = 0; k < kp; ++k) {
+        const double v = k < d ? (X[(int64_t)i * d + k] - mean[k]) * sc : 0.0;
+        if (k < d) xs[(int64_t)i * d + k] = v;
+        acc = fma(v, v, acc);
+        const __bf16 h = (__bf16)(float)v;
+        const double r1 = v - (double)(float)h;
+        const __bf16 m = (__bf16)(float)r1;
+        const double r2 = r1 - (double)(float)m;
+        p0[(int64_t)i * kp + k] = h;
+        p1[(int64_t)i * kp + k] = m;
+        if (npiece > 2) p2[(int64_t)i * kp + k] = (__bf16)(float)r2;
+    }
+    norm[i] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// exact median of D_ij = sum_k (x_ik - y_jk)^2 without storing D: each pass
+// regenerates 64 x 64 tiles in FP64 and histograms the keys matching the
+// prefix found so far (same digit schedule as the stored-matrix select)
+// ---------------------------------------------------------------------------
+struct FzSelect {
+    unsigned long long prefix, mask;
+    long long rank;
+    unsigned long long eq, next;
+    double median;
+};
+
+template <int MODE>  // 0: histogram digit, 1: smallest key above the prefix
+__global__ __launch_bounds__(256) void fz_select_kernel(const double *__restrict__ X, const double *__restrict__ Y,
+                                                        int n_i, int n_j, int d, int shift, int width,
+                                                        FzSelect *__restrict__ st,
+                                                        unsigned long long *__restrict__ ghist) {
+    __shared__ double xs[32][64];
+    __shared__ double ys[32][64];
+    __shared__ unsigned int h[2048];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+    if (MODE == 0)
+        for (int b = tid; b < 2048; b += 256) h[b] = 0u;
+    double acc[4][4] = {};
+    for (int k0 = 0; k0 < d; k0 += 32) {
+        for (int e = tid; e < 64 * 32; e += 256) {
+            const int r = e / 32, kk = e % 32;
+            const bool kin = k0 + kk < d;
+            xs[kk][r] = (kin && i0 + r < n_i) ? X[(int64_t)(i0 + r) * d + k0 + kk] : 0.0;
+            ys[kk][r] = (kin && j0 + r < n_j) ? Y[(int64_t)(j0 + r) * d + k0 + kk] : 0.0;
+        }
+        __syncthreads();
+        const int km = min(32, d - k0);
+        for (int kk = 0; kk < km; ++kk)
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const double t = xs[kk][ty * 4 + a] - ys[kk][tx * 4 + b];
+                    acc[a][b] = fma(t, t, acc[a][b]);
+                }
+        __syncthreads();
+    }
+    const unsigned long long prefix = st->prefix, mask = st->mask;
+    unsigned long long best = ~0ull;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            if (i0 + ty * 4 + a >= n_i || j0 + tx * 4 + b >= n_j) continue;
+            const unsigned long long key = (unsigned long long)__double_as_longlong(acc[a][b]);
+            if (MODE == 0) {
+                if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & ((1ull << width) - 1ull)], 1u);
+            } else {
+                if (key > prefix && key < best) best = key;
+            }
+        }
+    if (MODE == 0) {
+        __syncthreads();
+        for (int b = tid; b < 2048; b += 256)
+            if (h[b]) atomicAdd(&ghist[b], (unsigned long long)h[b]);
+    } else {
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(best, off);
+            best = o < best ? o : best;
+        }
+        if ((tid & 63) == 0 && best != ~0ull) atomicMin(&st->next, best);
+    }
+}
+
+__global__ void fz_select_init_kernel(FzSelect *st, long long rank, unsigned long long *hist) {
+    for (int b = threadIdx.x; b < 2048; b += blockDim.x) hist[b] = 0ull;
+    if (threadIdx.x) return;
+    st->prefix = 0ull;
+    st->mask = 0ull;
+    st->rank = rank;
+    st->eq = 0ull;
+    st->next = ~0ull;
+}
+
+__global__ __launch_bounds__(256) void fz_select_scan_kernel(unsigned long long *__restrict__ ghist, int shift,
+                                                             int width, int last, FzSelect *__restrict__ st) {
+    if (threadIdx.x == 0) {
+        const int nb = 1 << width;
+        long long rank = st->rank;
+        unsigned long long cum = 0;
+        int b = 0;
+        for (; b < nb - 1; ++b) {
+            if (cum + ghist[b] > (unsigned long long)rank) break;
+            cum += ghist[b];
+        }
+        st->prefix |= (unsigned long long)b << shift;
+        st->mask |= ((1ull << width) - 1ull) << shift;
+        st->rank = rank - (long long)cum;
+        if (last) st->eq = ghist[b];
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 2048; b += blockDim.x) ghist[b] = 0ull;
+}
+
+__global__ void fz_select_finish_kernel(FzSelect *st, int odd) {
+    const double lo = __longlong_as_double((long long)st->prefix);
+    double m = lo;
+    if (!odd) {
+        const bool tie = (unsigned long long)(st->rank + 1) < st->eq;
+        m = (lo + (tie ? lo : __longlong_as_double((long long)st->next))) / 2.0;
+    }
+    st->median = m;
+}
+
+// ---------------------------------------------------------------------------
+// fused sweep: row LSE over recomputed cost tiles, potential update epilogue
+// ---------------------------------------------------------------------------
+struct FusedArgs {
+    const __bf16 *A0, *A1, *A2;  // row points, pitch kp
+    const __bf16 *B0, *B1, *B2;  // column points, pitch kp
+    const double *na, *nb;       // squared norms of the scaled coordinates
+    int n_rows, n_cols, kp;
+    const double *h;             // column potentials
+    double *pot;                 // row potentials (read as the shift, then updated)
+    const double *logw;          // per-row log weight or nullptr -> *logw_dev
+    const double *logw_dev;
+    double log_n_other, alpha, eps, k;
+    int *bad_list;               // rows whose fixed-shift sum under/overflowed
+    int *n_bad;
+};
+
+template <int TERMS, int NKS>
+__global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
+    constexpr int NP = TERMS == 3 ? 2 : 3;  // pieces per point
+    constexpr int KP = NKS * 16;
+    constexpr int PITCH = KP + 8;            // 16-byte pad: spreads lanes over LDS banks
+    constexpr int TILE = NP * 32 * PITCH;
+    __shared__ __bf16 bsh[2 * TILE];         // double-buffered column tile
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 31, kg = (lane >> 5) * 8;
+    const int r0 = blockIdx.x * 128 + wave * 32;
+    const __bf16 *Ap[3] = {fa.A0, fa.A1, fa.A2};
+    const __bf16 *Bp[3] = {fa.B0, fa.B1, fa.B2};
+
+    // row constants for this lane's 16 accumulator rows: (f_old - |x'|^2) * k
+    double wf[16];
+    float s[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = min(r0 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4), fa.n_rows - 1);
+        wf[r] = (fa.pot[row] - fa.na[row]) * fa.k;
+        s[r] = 0.f;
+    }
+    // this wave's 32 rows stay in registers for the whole sweep
+    const int arow = min(r0 + lr, fa.n_rows - 1);
+    bf16x8 afr[NKS][NP];
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+            afr[ks][p] = *reinterpret_cast<const bf16x8 *>(Ap[p] + (int64_t)arow * KP + ks * 16 + kg);
+    const double k2 = 2.0 * fa.k;
+
+    auto load_tile = [&](int buf, int c0) {
+        // NP x 32 columns x KP bf16 -> LDS, 8 elements per thread-step
+        constexpr int PER_PIECE = 32 * KP / 8;
+        for (int e = tid; e < NP * PER_PIECE; e += kFThreads) {
+            const int p = e / PER_PIECE, rem = e % PER_PIECE;
+            const int col = rem / (KP / 8), kk = (rem % (KP / 8)) * 8;
+            const int gcol = min(c0 + col, fa.n_cols - 1);
+            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(Bp[p] + (int64_t)gcol * KP + kk);
+            *reinterpret_cast<bf16x8 *>(bsh + buf * TILE + (p * 32 + col) * PITCH + kk) = v;
+        }
+    };
+
+    const int ntiles = (fa.n_cols + 31) / 32;
+    load_tile(0, 0);
+    __syncthreads();
+    for (int t = 0; t < ntiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ntiles) load_tile(buf ^ 1, (t + 1) * 32);
+        floatx16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+        const __bf16 *bt = bsh + buf * TILE;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+            const int ko = ks * 16 + kg;
+            bf16x8 b[NP];
+#pragma unroll
+            for (int p = 0; p < NP; ++p) b[p] = *reinterpret_cast<const bf16x8 *>(bt + (p * 32 + lr) * PITCH + ko);
+            const bf16x8 *a = afr[ks];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[0], acc, 0, 0, 0);
+            if (TERMS == 6) {
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[NP - 1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[NP - 1], b[0], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[1], acc, 0, 0, 0);
+            }
+        }
+        const int col = t * 32 + lr;
+        if (col < fa.n_cols) {
+            const double uc = (fa.h[col] - fa.nb[col]) * fa.k;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) s[r] += __builtin_amdgcn_exp2f((float)fma(k2, (double)acc[r], uc + wf[r]));
+        }
+        __syncthreads();
+    }
+    // sum over the 32 lanes that share a row set
+    double sd[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        double v = (double)s[r];
+#pragma unroll
+        for (int off = 1; off < 32; off <<= 1) v += __shfl_xor(v, off);
+        sd[r] = v;
+    }
+    if (lr == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = r0 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+            if (row >= fa.n_rows) continue;
+            const double S = sd[r];
+            const double f_old = fa.pot[row];
+            if (!(S > 0.0) || isinf(S)) {
+                if (!isnan(S)) fa.bad_list[atomicAdd(fa.n_bad, 1)] = row;
+                continue;
+            }
+            const double lse = log(S) - f_old / fa.eps;  // ln sum_j exp((h_j - C_ij)/eps)
+            const double lw = fa.logw ? fa.logw[row] : *fa.logw_dev;
+            fa.pot[row] = fa.alpha * fa.eps * (lw + fa.log_n_other - lse);
+        }
+    }
+}
+
+// repair: rows whose fixed-shift sum under/overflowed, from FP64 direct
+// distances of the scaled coordinates with a per-lane online maximum
+__global__ __launch_bounds__(256) void fused_repair_kernel(const int *__restrict__ bad_list, const int *__restrict__ n_bad,
+                                                           const double *__restrict__ xr, const double *__restrict__ xc,
+                                                           int n_cols, int d, const double *__restrict__ h,
+                                                           double *__restrict__ pot, const double *__restrict__ logw,
+                                                           const double *__restrict__ logw_dev, double log_n_other,
+                                                           double alpha, double eps, double k) {
+    __shared__ double sg_sh[4], sd_sh[4];
+    const int nb = *n_bad;
+    for (int q = blockIdx.x; q < nb; q += gridDim.x) {
+        const int row = bad_list[q];
+        double sg = -1.0e6, sd = 0.0;
+        for (int j = threadIdx.x; j < n_cols; j += blockDim.x) {
+            double c = 0.0;
+            for (int t = 0; t < d; ++t) {
+                const double z = xr[(int64_t)row * d + t] - xc[(int64_t)j * d + t];
+                c = fma(z, z, c);
+            }
+            const double y = (h[j] - c) * k;
+            if (y > sg) {
+                const double ns = ceil(y);
+                sd = ldexp(sd, (int)(sg - ns));
+                sg = ns;
+            }
+            sd += exp2(y - sg);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double og = __shfl_xor(sg, off), os = __shfl_xor(sd, off);
+            const double ng = fmax(sg, og);
+            sd = ldexp(sd, (int)(sg - ng)) + ldexp(os, (int)(og - ng));
+            sg = ng;
+        }
+        if ((threadIdx.x & 63) == 0) {
+            sg_sh[threadIdx.x >> 6] = sg;
+            sd_sh[threadIdx.x >> 6] = sd;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double g = sg_sh[0], s = sd_sh[0];
+            for (int w = 1; w < 4; ++w) {
+                const double ng = fmax(g, sg_sh[w]);
+                s = ldexp(s, (int)(g - ng)) + ldexp(sd_sh[w], (int)(sg_sh[w] - ng));
+                g = ng;
+            }
+            const double lse = (g + log2(s)) * kLn2;
+            const double lw = logw ? logw[row] : *logw_dev;
+            pot[row] = alpha * eps * (lw + log_n_other - lse);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void fused_clear_kernel(int *n_bad) { *n_bad = 0; }
+
+__global__ __launch_bounds__(1024) void fz_growth_kernel(const double *__restrict__ G, int n, double *__restrict__ logp,
+                                                         double *__restrict__ misc) {
+    __shared__ double red[16];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        logp[i] = log(G[i]);
+        acc += G[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+        misc[0] = log(t / n);
+    }
+}
+
+__global__ void fz_zero_kernel(double *x, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = 0.0;
+}
+
+struct FzWork {
+    double *mean, *xs, *ys, *nx, *ny, *f, *g, *logp, *misc;
+    __bf16 *xp[3], *yp[3];
+    int *bad, *nbad;
+    FzSelect *sel;
+    unsigned long long *hist;
+};
+
+static size_t fz_carve(Arena &ar, int64_t I, int64_t J, int64_t d, FzWork *w) {
+    const int64_t kp = round_up(d, 16);
+    w->mean = ar.take<double>(d);
+    w->xs = ar.take<double>((size_t)I * d);
+    w->ys = ar.take<double>((size_t)J * d);
+    w->nx = ar.take<double>(I);
+    w->ny = ar.take<double>(J);
+    w->f = ar.take<double>(I);
+    w->g = ar.take<double>(J);
+    w->logp = ar.take<double>(I);
+    w->misc = ar.take<double>(8);
+    for (int p = 0; p < 3; ++p) {
+        w->xp[p] = ar.take<__bf16>((size_t)I * kp);
+        w->yp[p] = ar.take<__bf16>((size_t)J * kp);
+    }
+    w->bad = ar.take<int>(std::max(I, J));
+    w->nbad = ar.take<int>(4);
+    w->sel = ar.take<FzSelect>(1);
+    w->hist = ar.take<unsigned long long>(2048);
+    return ar.off;
+}
+
+static const int kFzShift[6] = {53, 42, 31, 20, 10, 0};
+static const int kFzWidth[6] = {11, 11, 11, 11, 10, 10};
+
+template <int TERMS>
+static void launch_fused(const FusedArgs &fa, hipStream_t s) {
+    const unsigned grid = (unsigned)((fa.n_rows + 127) / 128);
+    switch (fa.kp / 16) {
+        case 1: fused_lse_kernel<TERMS, 1><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 2: fused_lse_kernel<TERMS, 2><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 3: fused_lse_kernel<TERMS, 3><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 4: fused_lse_kernel<TERMS, 4><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 5: fused_lse_kernel<TERMS, 5><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 6: fused_lse_kernel<TERMS, 6><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 7: fused_lse_kernel<TERMS, 7><<<grid, kFThreads, 0, s>>>(fa); break;
+        default: fused_lse_kernel<TERMS, 8><<<grid, kFThreads, 0, s>>>(fa); break;
+    }
+}
+
+template <int TERMS>
+static int fused_sweep(const FusedArgs &fa, const double *xr, const double *xc, int d, hipStream_t s) {
+    fused_clear_kernel<<<1, 1, 0, s>>>(fa.n_bad);
+    launch_fused<TERMS>(fa, s);
+    fused_repair_kernel<<<64, 256, 0, s>>>(fa.bad_list, fa.n_bad, xr, xc, fa.n_cols, d, fa.h, fa.pot, fa.logw,
+                                           fa.logw_dev, fa.log_n_other, fa.alpha, fa.eps, fa.k);
+    WOTK_LAUNCH_CHECK();
+    return WOTK_OK;
+}
+
+}  // namespace wotk
+
+using namespace wotk;
+
+extern "C" size_t wotk_fused_workspace_size(int64_t I, int64_t J, int64_t d) {
+    Arena ar(nullptr, 0);
+    FzWork w;
+    return fz_carve(ar, I, J, d, &w);
+}
+
+extern "C" int wotk_transport_stablev2_coords(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
+                                              const double *G, double lambda1, double lambda2, double epsilon,
+                                              int64_t scaling_iter, double tau, double epsilon0, int64_t extra_iter,
+                                              int64_t inner_iter_max, int32_t terms, double *f_out, double *g_out,
+                                              double *median_out, void *workspace, size_t workspace_bytes,
+                                              wotk_stats *stats, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    WOTK_REQUIRE(I > 0 && J > 0 && d > 0 && I < (1ll << 30) && J < (1ll << 30), "bad shape");
+    WOTK_REQUIRE(d <= 128, "the fused solver keeps a 128-dimensional row tile in registers: d <= 128");
+    WOTK_REQUIRE(terms == 3 || terms == 6, "terms must be 3 or 6");
+    WOTK_REQUIRE(scaling_iter >= 0 && extra_iter >= 0 && scaling_iter + extra_iter > 0 && inner_iter_max > 0,
+                 "bad iteration counts");
+    (void)tau;
+    Arena ar(workspace, workspace_bytes);
+    FzWork w;
+    fz_carve(ar, I, J, d, &w);
+    if (!workspace || !ar.ok) return fail(WOTK_EWORKSPACE, "fused workspace too small");
+    const int kp = (int)round_up(d, 16);
+
+    // exact median of the FP64 distances, regenerated per pass
+    const long long n = (long long)I * J;
+    const long long rank = (n % 2) ? n / 2 : n / 2 - 1;
+    fz_select_init_kernel<<<1, 256, 0, s>>>(w.sel, rank, w.hist);
+    dim3 sgrid((unsigned)((J + 63) / 64), (unsigned)((I + 63) / 64));
+    for (int p = 0; p < 6; ++p) {
+        fz_select_kernel<0><<<sgrid, 256, 0, s>>>(X, Y, (int)I, (int)J, (int)d, kFzShift[p], kFzWidth[p], w.sel, w.hist);
+        fz_select_scan_kernel<<<1, 256, 0, s>>>(w.hist, kFzShift[p], kFzWidth[p], p == 5, w.sel);
+    }
+    if (n % 2 == 0) fz_select_kernel<1><<<sgrid, 256, 0, s>>>(X, Y, (int)I, (int)J, (int)d, 0, 0, w.sel, w.hist);
+    fz_select_finish_kernel<<<1, 1, 0, s>>>(w.sel, (int)(n % 2));
+    WOTK_LAUNCH_CHECK();
+    const double *med = &w.sel->median;
+    if (median_out) WOTK_HIP(hipMemcpyAsync(median_out, med, sizeof(double), hipMemcpyDeviceToDevice, s));
+
+    // scaled, centred coordinates and their bf16 pieces
+    fz_mean_kernel<<<(unsigned)d, 256, 0, s>>>(X, (int)I, Y, (int)J, (int)d, w.mean);
+    const int np = terms == 3 ? 2 : 3;
+    fz_split_kernel<<<(unsigned)((I + 255) / 256), 256, 0, s>>>(X, (int)I, (int)d, kp, w.mean, med, np, w.xp[0],
+                                                                w.xp[1], w.xp[2], w.xs, w.nx);
+    fz_split_kernel<<<(unsigned)((J + 255) / 256), 256, 0, s>>>(Y, (int)J, (int)d, kp, w.mean, med, np, w.yp[0],
+                                                                w.yp[1], w.yp[2], w.ys, w.ny);
+    fz_zero_kernel<<<64, 256, 0, s>>>(w.f, (int)I);
+    fz_zero_kernel<<<64, 256, 0, s>>>(w.g, (int)J);
+    fz_growth_kernel<<<1, 1024, 0, s>>>(G, (int)I, w.logp, w.misc);
+    WOTK_LAUNCH_CHECK();
+
+    auto reg = [&](int64_t m) { return (epsilon0 - epsilon) * std::exp(-(double)m) + epsilon; };
+    const double lnJ = std::log((double)J), lnI = std::log((double)I);
+    hipEvent_t ev[2];
+    WOTK_HIP(hipEventCreate(&ev[0]));
+    WOTK_HIP(hipEventCreate(&ev[1]));
+    WOTK_HIP(hipEventRecord(ev[0], s));
+    const int64_t n_iter = scaling_iter + extra_iter;
+    double eps = epsilon0;
+    for (int64_t it = 0; it < n_iter; ++it) {
+        eps = it < scaling_iter ? reg(it / inner_iter_max) : reg(scaling_iter / inner_iter_max);
+        const double k = kLog2E / eps;
+        FusedArgs fr{w.xp[0], w.xp[1], w.xp[2], w.yp[0], w.yp[1], w.yp[2], w.nx, w.ny, (int)I, (int)J, kp,
+                     w.g, w.f, w.logp, nullptr, lnJ, lambda1 / (lambda1 + eps), eps, k, w.bad, w.nbad};
+        FusedArgs fc{w.yp[0], w.yp[1], w.yp[2], w.xp[0], w.xp[1], w.xp[2], w.ny, w.nx, (int)J, (int)I, kp,
+                     w.f, w.g, nullptr, w.misc, lnI, lambda2 / (lambda2 + eps), eps, k, w.bad, w.nbad};
+        int rc = terms == 3 ? fused_sweep<3>(fr, w.xs, w.ys, (int)d, s) : fused_sweep<6>(fr, w.xs, w.ys, (int)d, s);
+        if (rc) return rc;
+        rc = terms == 3 ? fused_sweep<3>(fc, w.ys, w.xs, (int)d, s) : fused_sweep<6>(fc, w.ys, w.xs, (int)d, s);
+        if (rc) return rc;
+    }
+    WOTK_HIP(hipEventRecord(ev[1], s));
+    if (n_iter > 0 && scaling_iter > 0 && n_iter == scaling_iter) eps = reg(scaling_iter / inner_iter_max);
+    if (f_out) WOTK_HIP(hipMemcpyAsync(f_out, w.f, sizeof(double) * I, hipMemcpyDeviceToDevice, s));
+    if (g_out) WOTK_HIP(hipMemcpyAsync(g_out, w.g, sizeof(double) * J, hipMemcpyDeviceToDevice, s));
+    WOTK_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    WOTK_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    if (stats) {
+        *stats = wotk_stats{};
+        stats->iterations = n_iter;
+        stats->sweeps = 2 * n_iter;
+        stats->sweep_ms = ms;
+        stats->epsilon_final = reg(scaling_iter / inner_iter_max);
+    }
+    return WOTK_OK;
+}

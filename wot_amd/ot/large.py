@@ -1,0 +1,53 @@
+"""Large single pairs without a stored cost: ``transport_stablev2_coords``
+runs the fixed_iters solve (wot/ot/optimal_transport.py:167-236) on
+C = sqeuclid(x, y) / median (wot/ot/ot_model.py:242-253) regenerating C from
+the coordinates in every sweep (libwotk ``wotk_transport_stablev2_coords``:
+split-BF16 MFMA cross term with ``terms`` = 3 or 6 products, FP64 potentials,
+exact FP64 median).  Memory is O(I + J): a 60,000 x 60,000 pair needs no cost
+matrix.  The result holds the potentials; plan entries are
+exp((f_i + g_j - C_ij)/eps) / J.
+"""
+import ctypes
+
+from .. import _native
+from . import device as _dev
+
+
+class CoordsResult:
+    def __init__(self, f, g, median, stats, eps_final):
+        self.f, self.g, self.median, self.stats, self.epsilon = f, g, median, stats, eps_final
+
+    def plan_rows(self, x, y, rows):
+        """Rows of the plan R / J (FP64, exact FP64 cost) for the given source indices."""
+        torch = _dev.torch_mod()
+        X = _dev.as_device_f64(torch, x, self.f.device)[rows]
+        Y = _dev.as_device_f64(torch, y, self.f.device)
+        D = _dev.sqeuclid_f64(X, Y)
+        C = D / self.median
+        return torch.exp((self.f[rows][:, None] + self.g[None, :] - C) / self.epsilon) / Y.shape[0]
+
+
+def transport_stablev2_coords(x, y, G, lambda1=1.0, lambda2=50.0, epsilon=0.05, scaling_iter=3000, tau=1e4,
+                              epsilon0=1.0, extra_iter=1000, inner_iter_max=50, terms=3, device='cuda', **ignored):
+    torch = _dev.torch_mod()
+    X = _dev.as_device_f64(torch, x, device)
+    Y = _dev.as_device_f64(torch, y, device)
+    Gd = _dev.as_device_f64(torch, G, device).view(-1)
+    n_i, d = X.shape
+    n_j = Y.shape[0]
+    if Y.shape[1] != d or Gd.numel() != n_i:
+        raise ValueError('shape mismatch')
+    L = _native.lib()
+    ws = _dev.workspace(torch, L.wotk_fused_workspace_size(n_i, n_j, d), device)
+    f = torch.empty(n_i, dtype=torch.float64, device=device)
+    g = torch.empty(n_j, dtype=torch.float64, device=device)
+    med = torch.empty(1, dtype=torch.float64, device=device)
+    st = _native.Stats()
+    p = _dev._ptr
+    _native.check(L.wotk_transport_stablev2_coords(p(X), p(Y), n_i, n_j, d, p(Gd), float(lambda1), float(lambda2),
+                                                   float(epsilon), int(scaling_iter), float(tau), float(epsilon0),
+                                                   int(extra_iter), int(inner_iter_max), int(terms), p(f), p(g), p(med),
+                                                   p(ws), ws.numel(), ctypes.byref(st),
+                                                   ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    stats = st.as_dict()
+    return CoordsResult(f, g, med, stats, stats['epsilon_final'])
