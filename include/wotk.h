@@ -168,16 +168,17 @@ int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pairs, int32_t
  * never stored: every sweep regenerates C from the coordinates on the matrix
  * cores (split-BF16 cross term, `terms` = 3 or 6 products, FP32 accumulate)
  * with FP64 potentials; the median is the exact median of the FP64 distances.
- * d <= 128.  f_out (I), g_out (J), median_out (1): device, may be NULL.
+ * d <= 128.  f_out (I), g_out (J), row_sums (I: row sums of the returned plan
+ * R / J, the growth loop's next G), median_out (1): device, may be NULL.
  * ------------------------------------------------------------------------- */
 size_t wotk_fused_workspace_size(int64_t I, int64_t J, int64_t d);
 int wotk_transport_stablev2_coords(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
                                    const double *G, double lambda1, double lambda2, double epsilon,
                                    int64_t scaling_iter, double tau, double epsilon0,
                                    int64_t extra_iter, int64_t inner_iter_max, int32_t terms,
-                                   double *f_out, double *g_out, double *median_out,
-                                   void *workspace, size_t workspace_bytes, wotk_stats *stats,
-                                   void *stream);
+                                   double *f_out, double *g_out, double *row_sums,
+                                   double *median_out, void *workspace, size_t workspace_bytes,
+                                   wotk_stats *stats, void *stream);
 
 /* Primal / dual objectives of optimal_transport.py:45-62 at the plan
  * R = exp((f_i + g_j - C_ij)/eps) (un-normalised), FP64 reductions.

@@ -62,3 +62,19 @@ def test_fused_odd_sizes_and_dims(mods):
     dp = day_pair(97, 131, dim=7, n_components=3, seed=5)
     err, _ = _compare(torch, ot, dev, large, dp.x, dp.y, dp.growth, 6, SCHED)
     assert err < 1e-5, err
+
+
+def test_fused_row_sums_and_growth_loop(mods):
+    """Row sums of the returned plan (the next growth iteration's G) and the
+    growth loop against the stored-cost solver's loop."""
+    torch, ot, dev, large = mods
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(300, 420, dim=30, seed=9)
+    res, learned = large.transport_map_coords(dp.x, dp.y, dp.growth, growth_iters=3, terms=6, **SCHED)
+    cost = ot.DeviceCost.from_coordinates(dp.x, dp.y)
+    R, lg = ot.compute_transport_matrix(solver=ot.transport_stablev2, C=cost, G=dp.growth, growth_iters=3, **SCHED)
+    want = R.sum(1)
+    got = res.row_sums.cpu().numpy()
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-6
+    for a, b in zip(learned[1:], lg[1:]):
+        assert np.abs(a.cpu().numpy() - b).max() / np.abs(b).max() < 1e-6

@@ -7,8 +7,9 @@
 // One launch per half-iteration sweeps all pairs and all combinations: a
 // persistent grid pulls (pair, row-block) tiles from a work queue ordered
 // longest-row first, and every cost entry read from HBM serves all E
-// combinations.  Potentials are stored combination-minor ([n][E]) so a lane
-// reads the E values of its columns contiguously.  The per-iteration
+// combinations.  Potentials are stored combination-major ([E][n padded to 4])
+// so, for each combination, a wave reads its columns' potentials as one
+// contiguous, coalesced stream.  The per-iteration
 // epsilon / alpha / k table lives in device memory and the kernels read the
 // iteration index from a device counter, so a block of iterations can be
 // captured once in a hipGraph and replayed.
@@ -33,8 +34,9 @@ struct PairDev {
     long long a_off;   // element offset of this pair's matrix (C or C^T) in the cost buffer
     long long lda;
     int nrows, ncols;
-    long long pot_off;    // offset of this side's potentials ([nrows][E]) in the pot buffer
-    long long other_off;  // offset of the other side's potentials ([ncols pad 4][E])
+    long long pot_off;    // offset of this side's potentials ([E][pot_ld]) in the pot buffer
+    long long other_off;  // offset of the other side's potentials ([E][other_ld])
+    int pot_ld, other_ld; // row counts padded to a multiple of 4
     long long logw_off;   // offset of log weights ([nrows]) or -1 for the pair's scalar
     int pair;
 };
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
             rowp[r] = A + (long long)row * pd.lda;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                fk[r][e] = pot[(long long)row * E + e] * kk[e];
+                fk[r][e] = pot[(long long)e * pd.pot_ld + row] * kk[e];
                 s[r][e] = 0.f;
             }
         }
@@ -130,12 +132,12 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
                 cv[r][2] = a.z;
                 cv[r][3] = a.w;
             }
-            const double *hc = h + (long long)c * E;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                double hk[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) hk[q] = hc[q * E + e] * kk[e];
+                const double *he = h + (long long)e * pd.other_ld + c;
+                const double2 h01 = *reinterpret_cast<const double2 *>(he);
+                const double2 h23 = *reinterpret_cast<const double2 *>(he + 2);
+                const double hk[4] = {h01.x * kk[e], h01.y * kk[e], h23.x * kk[e], h23.y * kk[e]};
 #pragma unroll
                 for (int r = 0; r < kBRows; ++r) {
                     float acc = 0.f;
@@ -149,10 +151,10 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
         for (int r = 0; r < kBRows; ++r)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                double v = (double)s[r][e];
+                float v = s[r][e];  // positive terms: an FP32 tree over 64 lanes loses < 4e-7
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if (lane == 0) red_a[r * E + e][wave] = v;
+                if (lane == 0) red_a[r * E + e][wave] = (double)v;
             }
         __syncthreads();
         if (tid < RE) {
@@ -176,7 +178,6 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
                     s[r][e] = 0.f;
                 }
             for (int c = tid * 4; c < pd.ncols; c += kBThreads * 4) {
-                const double *hc = h + (long long)c * E;
 #pragma unroll
                 for (int r = 0; r < kBRows; ++r) {
                     const float4 a = bload4(rowp[r] + c);
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
                     for (int e = 0; e < E; ++e)
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
-                            const double y = fma(-kk[e], av[q], hc[q * E + e] * kk[e]);
+                            const double y = fma(-kk[e], av[q], h[(long long)e * pd.other_ld + c + q] * kk[e]);
                             if (y > sig[r][e]) {
                                 const double ns = ceil(y);
                                 s[r][e] = ldexpf(s[r][e], (int)(sig[r][e] - ns));
@@ -234,7 +235,7 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
                 const double lse = l2_sh[tid] * kLn2;  // ln sum_j exp((h_j - C_ij)/eps)
                 const double lw = (pd.logw_off >= 0) ? ba.logw[pd.logw_off + row] : ba.logq[pd.pair];
                 const double alpha = ba.side == 0 ? ip.alpha1 : ip.alpha2;
-                pot[(long long)row * E + e] = alpha * ip.eps * (lw + ba.log_n_other[tl.side_pair] - lse);
+                pot[(long long)e * pd.pot_ld + row] = alpha * ip.eps * (lw + ba.log_n_other[tl.side_pair] - lse);
             }
         }
         __syncthreads();
@@ -279,9 +280,16 @@ __global__ __launch_bounds__(256) void batch_growth_kernel(const double *__restr
     if (threadIdx.x == 0) logq[p] = log((red[0] + red[1] + red[2] + red[3]) / n);
 }
 
-// potentials of one (pair, combination) from [n][E] to a contiguous vector
-__global__ void batch_gather_kernel(const double *__restrict__ pot, int n, int E, int e, double *__restrict__ out) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = pot[(long long)i * E + e];
+// -inf on the padding rows [n, n_pad) of every (pair, combination) block:
+// read as "other" potentials they make the padded columns contribute 0
+__global__ void batch_pad_kernel(double *__restrict__ pot, const PairDev *__restrict__ pairs, int n_pairs, int E) {
+    const int p = blockIdx.x;
+    if (p >= n_pairs) return;
+    const PairDev pd = pairs[p];
+    for (int t = threadIdx.x; t < E * 4; t += blockDim.x) {
+        const int e = t / 4, i = pd.nrows + t % 4;
+        if (i < pd.pot_ld) pot[pd.pot_off + (long long)e * pd.pot_ld + i] = -INFINITY;
+    }
 }
 
 struct BatchWork {
@@ -418,8 +426,9 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     long long of = 0, og = 0, olw = 0;
     for (int p = 0; p < n_pairs; ++p) {
         const wotk_batch_pair &q = pairs[p];
-        pf[p] = PairDev{q.c_offset, q.ldc, (int)q.n_i, (int)q.n_j, of * E, og * E, olw, p};
-        pg[p] = PairDev{q.ct_offset, q.ldct, (int)q.n_j, (int)q.n_i, og * E, of * E, -1, p};
+        const int ldi = (int)round_up(q.n_i, 4), ldj = (int)round_up(q.n_j, 4);
+        pf[p] = PairDev{q.c_offset, q.ldc, (int)q.n_i, (int)q.n_j, of * E, og * E, ldi, ldj, olw, p};
+        pg[p] = PairDev{q.ct_offset, q.ldct, (int)q.n_j, (int)q.n_i, og * E, of * E, ldj, ldi, -1, p};
         goff[p] = q.g_offset;
         lwoff[p] = olw;
         nsrc[p] = (int)q.n_i;
@@ -462,18 +471,9 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     batch_fill_kernel<<<1024, 256, 0, s>>>(w.f, nf * E, 0.0);
     batch_fill_kernel<<<1024, 256, 0, s>>>(w.g, ng * E, 0.0);
     WOTK_LAUNCH_CHECK();
-    {
-        // -inf on the padding rows of each side (read as "other" potentials)
-        std::vector<double> ninf(4 * E, -INFINITY);
-        long long a = 0, b = 0;
-        for (int p = 0; p < n_pairs; ++p) {
-            const int64_t pi = round_up(pairs[p].n_i, 4) - pairs[p].n_i, pj = round_up(pairs[p].n_j, 4) - pairs[p].n_j;
-            if (pi) WOTK_HIP(hipMemcpyAsync(w.f + (a + pairs[p].n_i) * E, ninf.data(), sizeof(double) * pi * E, hipMemcpyHostToDevice, s));
-            if (pj) WOTK_HIP(hipMemcpyAsync(w.g + (b + pairs[p].n_j) * E, ninf.data(), sizeof(double) * pj * E, hipMemcpyHostToDevice, s));
-            a += round_up(pairs[p].n_i, 4);
-            b += round_up(pairs[p].n_j, 4);
-        }
-    }
+    batch_pad_kernel<<<n_pairs, 64, 0, s>>>(w.f, w.pairs_f, n_pairs, E);
+    batch_pad_kernel<<<n_pairs, 64, 0, s>>>(w.g, w.pairs_g, n_pairs, E);
+    WOTK_LAUNCH_CHECK();
     batch_growth_kernel<<<n_pairs, 256, 0, s>>>(G, w.g_off, w.n_src, w.logw, w.lw_off, w.logq);
     batch_reset_kernel<<<1, 1, 0, s>>>(w.iter, w.queues, 0);
     WOTK_LAUNCH_CHECK();
@@ -520,18 +520,21 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     WOTK_HIP(hipEventRecord(ev[1], s));
     // outputs: [pair][combo][n] contiguous blocks
     {
-        long long a = 0, b = 0, oa = 0, ob = 0;
+        long long oa = 0, ob = 0;
         for (int p = 0; p < n_pairs; ++p) {
             for (int e = 0; e < E; ++e) {
-                if (f_out) batch_gather_kernel<<<64, 256, 0, s>>>(w.f + a * E, (int)pairs[p].n_i, E, e, f_out + oa + (long long)e * pairs[p].n_i);
-                if (g_out) batch_gather_kernel<<<64, 256, 0, s>>>(w.g + b * E, (int)pairs[p].n_j, E, e, g_out + ob + (long long)e * pairs[p].n_j);
+                if (f_out)
+                    WOTK_HIP(hipMemcpyAsync(f_out + oa + (long long)e * pairs[p].n_i,
+                                            w.f + pf[p].pot_off + (long long)e * pf[p].pot_ld,
+                                            sizeof(double) * pairs[p].n_i, hipMemcpyDeviceToDevice, s));
+                if (g_out)
+                    WOTK_HIP(hipMemcpyAsync(g_out + ob + (long long)e * pairs[p].n_j,
+                                            w.g + pg[p].pot_off + (long long)e * pg[p].pot_ld,
+                                            sizeof(double) * pairs[p].n_j, hipMemcpyDeviceToDevice, s));
             }
-            a += round_up(pairs[p].n_i, 4);
-            b += round_up(pairs[p].n_j, 4);
             oa += (long long)E * pairs[p].n_i;
             ob += (long long)E * pairs[p].n_j;
         }
-        WOTK_LAUNCH_CHECK();
     }
     WOTK_HIP(hipStreamSynchronize(s));
     float ms = 0.f;

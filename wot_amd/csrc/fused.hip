@@ -193,6 +193,7 @@ struct FusedArgs {
     double log_n_other, alpha, eps, k;
     int *bad_list;               // rows whose fixed-shift sum under/overflowed
     int *n_bad;
+    double *row_sums;            // row-sum mode: write sum_j R_ij / n_cols, leave pot unchanged
 };
 
 template <int TERMS, int NKS>
@@ -293,6 +294,10 @@ __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
                 if (!isnan(S)) fa.bad_list[atomicAdd(fa.n_bad, 1)] = row;
                 continue;
             }
+            if (fa.row_sums) {  // the shift is the final potential: S = sum_j R_ij
+                fa.row_sums[row] = S / fa.n_cols;
+                continue;
+            }
             const double lse = log(S) - f_old / fa.eps;  // ln sum_j exp((h_j - C_ij)/eps)
             const double lw = fa.logw ? fa.logw[row] : *fa.logw_dev;
             fa.pot[row] = fa.alpha * fa.eps * (lw + fa.log_n_other - lse);
@@ -307,7 +312,8 @@ __global__ __launch_bounds__(256) void fused_repair_kernel(const int *__restrict
                                                            int n_cols, int d, const double *__restrict__ h,
                                                            double *__restrict__ pot, const double *__restrict__ logw,
                                                            const double *__restrict__ logw_dev, double log_n_other,
-                                                           double alpha, double eps, double k) {
+                                                           double alpha, double eps, double k,
+                                                           double *__restrict__ row_sums) {
     __shared__ double sg_sh[4], sd_sh[4];
     const int nb = *n_bad;
     for (int q = blockIdx.x; q < nb; q += gridDim.x) {
@@ -346,8 +352,12 @@ __global__ __launch_bounds__(256) void fused_repair_kernel(const int *__restrict
                 g = ng;
             }
             const double lse = (g + log2(s)) * kLn2;
-            const double lw = logw ? logw[row] : *logw_dev;
-            pot[row] = alpha * eps * (lw + log_n_other - lse);
+            if (row_sums) {
+                row_sums[row] = exp(pot[row] / eps + lse) / n_cols;
+            } else {
+                const double lw = logw ? logw[row] : *logw_dev;
+                pot[row] = alpha * eps * (lw + log_n_other - lse);
+            }
         }
         __syncthreads();
     }
@@ -430,7 +440,7 @@ static int fused_sweep(const FusedArgs &fa, const double *xr, const double *xc, 
     fused_clear_kernel<<<1, 1, 0, s>>>(fa.n_bad);
     launch_fused<TERMS>(fa, s);
     fused_repair_kernel<<<64, 256, 0, s>>>(fa.bad_list, fa.n_bad, xr, xc, fa.n_cols, d, fa.h, fa.pot, fa.logw,
-                                           fa.logw_dev, fa.log_n_other, fa.alpha, fa.eps, fa.k);
+                                           fa.logw_dev, fa.log_n_other, fa.alpha, fa.eps, fa.k, fa.row_sums);
     WOTK_LAUNCH_CHECK();
     return WOTK_OK;
 }
@@ -449,8 +459,8 @@ extern "C" int wotk_transport_stablev2_coords(const double *X, const double *Y, 
                                               const double *G, double lambda1, double lambda2, double epsilon,
                                               int64_t scaling_iter, double tau, double epsilon0, int64_t extra_iter,
                                               int64_t inner_iter_max, int32_t terms, double *f_out, double *g_out,
-                                              double *median_out, void *workspace, size_t workspace_bytes,
-                                              wotk_stats *stats, void *stream) {
+                                              double *row_sums, double *median_out, void *workspace,
+                                              size_t workspace_bytes, wotk_stats *stats, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     WOTK_REQUIRE(I > 0 && J > 0 && d > 0 && I < (1ll << 30) && J < (1ll << 30), "bad shape");
     WOTK_REQUIRE(d <= 128, "the fused solver keeps a 128-dimensional row tile in registers: d <= 128");
@@ -503,16 +513,23 @@ extern "C" int wotk_transport_stablev2_coords(const double *X, const double *Y, 
         eps = it < scaling_iter ? reg(it / inner_iter_max) : reg(scaling_iter / inner_iter_max);
         const double k = kLog2E / eps;
         FusedArgs fr{w.xp[0], w.xp[1], w.xp[2], w.yp[0], w.yp[1], w.yp[2], w.nx, w.ny, (int)I, (int)J, kp,
-                     w.g, w.f, w.logp, nullptr, lnJ, lambda1 / (lambda1 + eps), eps, k, w.bad, w.nbad};
+                     w.g, w.f, w.logp, nullptr, lnJ, lambda1 / (lambda1 + eps), eps, k, w.bad, w.nbad, nullptr};
         FusedArgs fc{w.yp[0], w.yp[1], w.yp[2], w.xp[0], w.xp[1], w.xp[2], w.ny, w.nx, (int)J, (int)I, kp,
-                     w.f, w.g, nullptr, w.misc, lnI, lambda2 / (lambda2 + eps), eps, k, w.bad, w.nbad};
+                     w.f, w.g, nullptr, w.misc, lnI, lambda2 / (lambda2 + eps), eps, k, w.bad, w.nbad, nullptr};
         int rc = terms == 3 ? fused_sweep<3>(fr, w.xs, w.ys, (int)d, s) : fused_sweep<6>(fr, w.xs, w.ys, (int)d, s);
         if (rc) return rc;
         rc = terms == 3 ? fused_sweep<3>(fc, w.ys, w.xs, (int)d, s) : fused_sweep<6>(fc, w.ys, w.xs, (int)d, s);
         if (rc) return rc;
     }
     WOTK_HIP(hipEventRecord(ev[1], s));
-    if (n_iter > 0 && scaling_iter > 0 && n_iter == scaling_iter) eps = reg(scaling_iter / inner_iter_max);
+    if (row_sums) {
+        // the returned plan is built with the eps after the last iteration
+        const double e_end = reg(scaling_iter / inner_iter_max);
+        FusedArgs fr{w.xp[0], w.xp[1], w.xp[2], w.yp[0], w.yp[1], w.yp[2], w.nx, w.ny, (int)I, (int)J, kp,
+                     w.g, w.f, w.logp, nullptr, lnJ, 0.0, e_end, kLog2E / e_end, w.bad, w.nbad, row_sums};
+        int rc = terms == 3 ? fused_sweep<3>(fr, w.xs, w.ys, (int)d, s) : fused_sweep<6>(fr, w.xs, w.ys, (int)d, s);
+        if (rc) return rc;
+    }
     if (f_out) WOTK_HIP(hipMemcpyAsync(f_out, w.f, sizeof(double) * I, hipMemcpyDeviceToDevice, s));
     if (g_out) WOTK_HIP(hipMemcpyAsync(g_out, w.g, sizeof(double) * J, hipMemcpyDeviceToDevice, s));
     WOTK_HIP(hipStreamSynchronize(s));

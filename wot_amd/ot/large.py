@@ -14,8 +14,9 @@ from . import device as _dev
 
 
 class CoordsResult:
-    def __init__(self, f, g, median, stats, eps_final):
+    def __init__(self, f, g, median, stats, eps_final, row_sums=None):
         self.f, self.g, self.median, self.stats, self.epsilon = f, g, median, stats, eps_final
+        self.row_sums = row_sums  # row sums of the returned plan R / J
 
     def plan_rows(self, x, y, rows):
         """Rows of the plan R / J (FP64, exact FP64 cost) for the given source indices."""
@@ -28,7 +29,8 @@ class CoordsResult:
 
 
 def transport_stablev2_coords(x, y, G, lambda1=1.0, lambda2=50.0, epsilon=0.05, scaling_iter=3000, tau=1e4,
-                              epsilon0=1.0, extra_iter=1000, inner_iter_max=50, terms=3, device='cuda', **ignored):
+                              epsilon0=1.0, extra_iter=1000, inner_iter_max=50, terms=3, want_row_sums=False,
+                              device='cuda', **ignored):
     torch = _dev.torch_mod()
     X = _dev.as_device_f64(torch, x, device)
     Y = _dev.as_device_f64(torch, y, device)
@@ -42,12 +44,31 @@ def transport_stablev2_coords(x, y, G, lambda1=1.0, lambda2=50.0, epsilon=0.05, 
     f = torch.empty(n_i, dtype=torch.float64, device=device)
     g = torch.empty(n_j, dtype=torch.float64, device=device)
     med = torch.empty(1, dtype=torch.float64, device=device)
+    rs = torch.empty(n_i, dtype=torch.float64, device=device) if want_row_sums else None
     st = _native.Stats()
     p = _dev._ptr
     _native.check(L.wotk_transport_stablev2_coords(p(X), p(Y), n_i, n_j, d, p(Gd), float(lambda1), float(lambda2),
                                                    float(epsilon), int(scaling_iter), float(tau), float(epsilon0),
-                                                   int(extra_iter), int(inner_iter_max), int(terms), p(f), p(g), p(med),
+                                                   int(extra_iter), int(inner_iter_max), int(terms), p(f), p(g), p(rs),
+                                                   p(med),
                                                    p(ws), ws.numel(), ctypes.byref(st),
                                                    ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
     stats = st.as_dict()
-    return CoordsResult(f, g, med, stats, stats['epsilon_final'])
+    return CoordsResult(f, g, med, stats, stats['epsilon_final'], rs)
+
+
+def transport_map_coords(x, y, G, growth_iters=1, **params):
+    """The growth-learning loop (optimal_transport.py:10-33) around the
+    never-materialised solver: iteration k > 0 solves with the previous
+    plan's row sums.  Returns (last CoordsResult, learned growth list)."""
+    torch = _dev.torch_mod()
+    learned = []
+    res = None
+    G_k = G
+    for it in range(growth_iters):
+        if it > 0:
+            G_k = res.row_sums
+        learned.append(G_k)
+        res = transport_stablev2_coords(x, y, G_k, want_row_sums=True, **params)
+    del torch
+    return res, learned
