@@ -570,8 +570,9 @@ def main():
         log('cpu sample')
         per_map_cpu, threads, cpu_detail = cpu_sample(dp.x, dp.y, dp.growth)
         out['cpu_baseline'] = {'value': 1.0 / per_map_cpu, 'unit': 'maps/s', 'cores': threads, 'kind': 'port',
-                               'sample': 'FP64 numpy oracle: full-size cost+median, 50- and 100-iteration '
-                                         'schedules at full size, extrapolated to the full map',
+                               'sample': 'FP64 numpy oracle at full size: cost+median, then 50 scheduled '
+                                         'iterations (2 kernel builds) and 50 extra iterations timed apart, '
+                                         'extrapolated to 3 x (4000 iterations + 61 builds)',
                                **cpu_detail}
         out['speedup_vs_cpu'] = maps_per_s * per_map_cpu
     print(json.dumps(out), flush=True)

@@ -228,27 +228,60 @@ __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
             afr[ks][p] = *reinterpret_cast<const bf16x8 *>(Ap[p] + (int64_t)arow * KP + ks * 16 + kg);
     const double k2 = 2.0 * fa.k;
 
-    auto load_tile = [&](int buf, int c0) {
-        // NP x 32 columns x KP bf16 -> LDS, 8 elements per thread-step
-        constexpr int PER_PIECE = 32 * KP / 8;
-        for (int e = tid; e < NP * PER_PIECE; e += kFThreads) {
-            const int p = e / PER_PIECE, rem = e % PER_PIECE;
-            const int col = rem / (KP / 8), kk = (rem % (KP / 8)) * 8;
-            const int gcol = min(c0 + col, fa.n_cols - 1);
-            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(Bp[p] + (int64_t)gcol * KP + kk);
-            *reinterpret_cast<bf16x8 *>(bsh + buf * TILE + (p * 32 + col) * PITCH + kk) = v;
+    // column tiles stream through two LDS buffers: the next tile's global
+    // loads are issued into registers before the current tile's MFMA and
+    // exponentials, and written to LDS only afterwards, so the load latency
+    // hides behind the compute
+    constexpr int PER_PIECE = 32 * KP / 8;                       // bf16x8 chunks per piece
+    constexpr int NCHUNK = (NP * PER_PIECE + kFThreads - 1) / kFThreads;
+    bf16x8 stage[NCHUNK];
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int u = 0; u < NCHUNK; ++u) {
+            const int e = tid + u * kFThreads;
+            if (e < NP * PER_PIECE) {
+                const int p = e / PER_PIECE, rem = e % PER_PIECE;
+                const int col = rem / (KP / 8), kk = (rem % (KP / 8)) * 8;
+                const int gcol = min(c0 + col, fa.n_cols - 1);
+                stage[u] = *reinterpret_cast<const bf16x8 *>(Bp[p] + (int64_t)gcol * KP + kk);
+            }
+        }
+    };
+    auto commit = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < NCHUNK; ++u) {
+            const int e = tid + u * kFThreads;
+            if (e < NP * PER_PIECE) {
+                const int p = e / PER_PIECE, rem = e % PER_PIECE;
+                const int col = rem / (KP / 8), kk = (rem % (KP / 8)) * 8;
+                *reinterpret_cast<bf16x8 *>(bsh + buf * TILE + (p * 32 + col) * PITCH + kk) = stage[u];
+            }
         }
     };
 
     const int ntiles = (fa.n_cols + 31) / 32;
-    load_tile(0, 0);
+    auto col_const = [&](int t, double &hv, double &nv) {  // this lane's column of tile t
+        const int col = min(t * 32 + lr, fa.n_cols - 1);
+        hv = fa.h[col];
+        nv = fa.nb[col];
+    };
+    double h_cur, n_cur, h_nxt = 0.0, n_nxt = 0.0;
+    col_const(0, h_cur, n_cur);
+    fetch(0);
+    commit(0);
     __syncthreads();
     for (int t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
-        if (t + 1 < ntiles) load_tile(buf ^ 1, (t + 1) * 32);
-        floatx16 acc;
+        if (t + 1 < ntiles) {
+            fetch((t + 1) * 32);
+            col_const(t + 1, h_nxt, n_nxt);
+        }
+        floatx16 acc, acc2;  // two independent accumulation chains
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+        for (int r = 0; r < 16; ++r) {
+            acc[r] = 0.f;
+            acc2[r] = 0.f;
+        }
         const __bf16 *bt = bsh + buf * TILE;
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks) {
@@ -258,20 +291,24 @@ __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
             for (int p = 0; p < NP; ++p) b[p] = *reinterpret_cast<const bf16x8 *>(bt + (p * 32 + lr) * PITCH + ko);
             const bf16x8 *a = afr[ks];
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[0], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[1], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[0], acc, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[1], acc2, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[0], acc2, 0, 0, 0);
             if (TERMS == 6) {
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[NP - 1], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[NP - 1], b[0], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[1], acc, 0, 0, 0);
+                acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[NP - 1], acc2, 0, 0, 0);
+                acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[NP - 1], b[0], acc2, 0, 0, 0);
+                acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[1], acc2, 0, 0, 0);
             }
         }
         const int col = t * 32 + lr;
         if (col < fa.n_cols) {
-            const double uc = (fa.h[col] - fa.nb[col]) * fa.k;
+            const double uc = (h_cur - n_cur) * fa.k;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) s[r] += __builtin_amdgcn_exp2f((float)fma(k2, (double)acc[r], uc + wf[r]));
+            for (int r = 0; r < 16; ++r)
+                s[r] += __builtin_amdgcn_exp2f((float)fma(k2, (double)acc[r] + (double)acc2[r], uc + wf[r]));
         }
+        if (t + 1 < ntiles) commit(buf ^ 1);  // the other buffer was released by the previous barrier
+        h_cur = h_nxt;
+        n_cur = n_nxt;
         __syncthreads();
     }
     // sum over the 32 lanes that share a row set
