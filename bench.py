@@ -153,12 +153,12 @@ def bench_timecourse(args):
     log('time course: %d pairs, %d combinations, sum I*J = %.3e, %d iterations' %
         (len(pair_ij), n_combo, sum_ij, n_iter))
     t0 = time.perf_counter()
-    res = batch.solve_time_course(coords, growth, graph_iters=50, **GRID, **sched)
+    res = batch.solve_time_course(coords, growth, graph_iters=50, group_size=args.group_size, **GRID, **sched)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     n_solves = len(pair_ij) * n_combo
     sweep_s = res.stats['sweep_ms'] / 1000.0
-    avg_sweep = sweep_s / (2 * n_iter)
+    avg_sweep = sweep_s / (2 * n_iter)  # per half-iteration over the whole grid (all groups)
     # algorithmic work of one sweep over the whole batch: 4 bytes per cost entry
     # (one read shared by all combinations) and one exp per (entry, combination)
     alg_bytes = 4.0 * sum_ij
@@ -464,6 +464,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--no-baselines', action='store_true', help='skip the plain-GPU and CPU samples')
     ap.add_argument('--large-n', type=int, default=60000, help='large only: cells per day')
+    ap.add_argument('--group-size', type=int, default=None,
+                    help='timecourse only: combinations per batched run (default: the whole grid)')
     ap.add_argument('--config', default='pair', choices=['pair', 'timecourse', 'adjoint', 'large'],
                     help='pair: one 15k x 15k map (default); timecourse: 38 pairs x 9-point grid; '
                          'adjoint: gradient of the converged objective, 8k x 8k')

@@ -94,10 +94,12 @@ def build_costs(coords, pairs_idx, cost_mode='f64', device='cuda'):
 
 def solve_time_course(coords, growth, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_iter=3000,
                       tau=1e4, epsilon0=1.0, extra_iter=1000, inner_iter_max=50, graph_iters=50,
-                      cost_mode='f64', pairs=None, device='cuda'):
+                      cost_mode='f64', pairs=None, group_size=None, device='cuda'):
     """Solve every consecutive day pair (or the given index pairs) for every
     (epsilon, lambda1) of the grid epsilons x lambda1s.  ``coords[t]`` are the
-    PCA coordinates of day t, ``growth[t]`` its cells' growth (G = growth, dt=1)."""
+    PCA coordinates of day t, ``growth[t]`` its cells' growth (G = growth, dt=1).
+    ``group_size`` splits the grid into batched runs of that many combinations
+    (each cost entry is then read once per group instead of once overall)."""
     torch = _dev.torch_mod()
     if pairs is None:
         pairs = [(t, t + 1) for t in range(len(coords) - 1)]
@@ -111,28 +113,40 @@ def solve_time_course(coords, growth, epsilons=(0.05,), lambda1s=(1.0,), lambda2
     arr = (_native.BatchPair * len(bps))(*bps)
     n_iter = scaling_iter + extra_iter
     L = _native.lib()
-    ws = _dev.workspace(torch, L.wotk_batch_workspace_size(arr, len(bps), E, n_iter), device)
-    nf = sum(E * s[0] for s in shapes)
-    ng = sum(E * s[1] for s in shapes)
-    f_out = torch.empty(nf, dtype=torch.float64, device=device)
-    g_out = torch.empty(ng, dtype=torch.float64, device=device)
-    eps = (ctypes.c_double * E)(*[c[0] for c in combos])
-    lam = (ctypes.c_double * E)(*[c[1] for c in combos])
-    st = _native.Stats()
-    _native.check(L.wotk_batch_stablev2(_dev._ptr(buf), arr, len(bps), _dev._ptr(G), E, eps, lam, float(lambda2),
-                                        int(scaling_iter), float(tau), float(epsilon0), int(extra_iter),
-                                        int(inner_iter_max), _dev._ptr(f_out), _dev._ptr(g_out), int(graph_iters),
-                                        _dev._ptr(ws), ws.numel(), ctypes.byref(st), _dev._stream(torch)))
-    fs, gs = [], []
-    a = b = 0
-    for (n_i, n_j) in shapes:
-        fs.append(f_out[a:a + E * n_i].view(E, n_i))
-        gs.append(g_out[b:b + E * n_j].view(E, n_j))
-        a += E * n_i
-        b += E * n_j
+    gsz = E if group_size is None else int(group_size)
+    groups = [list(range(k, min(E, k + gsz))) for k in range(0, E, gsz)]
+    ws = _dev.workspace(torch, max(L.wotk_batch_workspace_size(arr, len(bps), len(g), n_iter) for g in groups),
+                        device)
+    fs = [torch.empty((E, s[0]), dtype=torch.float64, device=device) for s in shapes]
+    gs = [torch.empty((E, s[1]), dtype=torch.float64, device=device) for s in shapes]
+    st_total = None
+    for grp in groups:
+        Eg = len(grp)
+        f_out = torch.empty(sum(Eg * s[0] for s in shapes), dtype=torch.float64, device=device)
+        g_out = torch.empty(sum(Eg * s[1] for s in shapes), dtype=torch.float64, device=device)
+        eps = (ctypes.c_double * Eg)(*[combos[e][0] for e in grp])
+        lam = (ctypes.c_double * Eg)(*[combos[e][1] for e in grp])
+        st = _native.Stats()
+        _native.check(L.wotk_batch_stablev2(_dev._ptr(buf), arr, len(bps), _dev._ptr(G), Eg, eps, lam,
+                                            float(lambda2), int(scaling_iter), float(tau), float(epsilon0),
+                                            int(extra_iter), int(inner_iter_max), _dev._ptr(f_out), _dev._ptr(g_out),
+                                            int(graph_iters), _dev._ptr(ws), ws.numel(), ctypes.byref(st),
+                                            _dev._stream(torch)))
+        a = b = 0
+        for k, (n_i, n_j) in enumerate(shapes):
+            fs[k][grp] = f_out[a:a + Eg * n_i].view(Eg, n_i)
+            gs[k][grp] = g_out[b:b + Eg * n_j].view(Eg, n_j)
+            a += Eg * n_i
+            b += Eg * n_j
+        d = st.as_dict()
+        if st_total is None:
+            st_total = d
+        else:
+            st_total['sweep_ms'] += d['sweep_ms']
+            st_total['sweeps'] += d['sweeps']
 
     def reg(e, n):
         return (epsilon0 - e) * np.exp(-n) + e
     n_end = scaling_iter // inner_iter_max
     eps_final = [reg(c[0], n_end) for c in combos]
-    return BatchResult(buf, bps, combos, fs, gs, st.as_dict(), shapes, eps_final)
+    return BatchResult(buf, bps, combos, fs, gs, st_total, shapes, eps_final)
