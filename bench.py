@@ -63,6 +63,18 @@ def one_map(ot, dev, x, y, growth):
     return cost, res, sweep_ms, sweeps
 
 
+def _tune_pair(args, dev, x, y, g):
+    """Autotune the row-sweep tile on this pair's cost (untimed)."""
+    if args.no_tune:
+        return None
+    from wot_amd import tune
+    cost = dev.DeviceCost.from_coordinates(x, y)
+    r = tune.autotune_row_sweep(cost, G=g, epsilon=SOLVER['epsilon'], lambda1=SOLVER['lambda1'],
+                                lambda2=SOLVER['lambda2'], persist=False)
+    log('tiles: %s' % json.dumps(r.as_dict()))
+    return r.as_dict()
+
+
 def plain_gpu_sample(torch, x, y, growth):
     """Plain GPU (torch FP64 dense K, one pair at a time): one full fixed_iters
     solve (4000 iterations, 61 kernel rebuilds) at full size, timed after a
@@ -150,6 +162,15 @@ def bench_timecourse(args):
         log('warmup (2 iterations)')
         batch.solve_time_course(coords[:3], growth[:3], scaling_iter=2, extra_iter=0, **GRID)
     torch.cuda.synchronize()
+    tiles = None
+    if not args.no_tune:
+        from wot_amd import tune
+        tc = batch.prepare_time_course(coords, growth)
+        tiles = tune.autotune_time_course(tc, GRID['epsilons'], GRID['lambda1s'], lambda2=sched['lambda2'],
+                                          persist=False).as_dict()
+        del tc
+        torch.cuda.empty_cache()
+        log('tiles: %s' % json.dumps(tiles))
     log('time course: %d pairs, %d combinations, sum I*J = %.3e, %d iterations' %
         (len(pair_ij), n_combo, sum_ij, n_iter))
     t0 = time.perf_counter()
@@ -168,7 +189,8 @@ def bench_timecourse(args):
         'ms_per_step': 1000.0 * elapsed, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f64/f32', 'data': 'synthetic (seeded drifting Gaussian mixture, 39 days)',
         'config': {'workload': '38-pair time course x eps{0.01,0.05,0.1} x lambda1{0.5,1,5}, fixed_iters',
-                   'cells_total': int(sum(counts)), 'sum_IJ': sum_ij, 'solves': n_solves, **sched},
+                   'cells_total': int(sum(counts)), 'sum_IJ': sum_ij, 'solves': n_solves, **sched,
+                   'tiles': tiles},
         'sinkhorn_iters_per_s': n_solves * n_iter / elapsed,
         'sweep_loop_s': sweep_s,
         'roofline': {'bound': 'valu/exp', 'kernel': 'batch_sweep_kernel<9>', 'avg_launch_us': avg_sweep * 1e6,
@@ -469,6 +491,8 @@ def main():
     ap.add_argument('--config', default='pair', choices=['pair', 'timecourse', 'adjoint', 'large'],
                     help='pair: one 15k x 15k map (default); timecourse: 38 pairs x 9-point grid; '
                          'adjoint: gradient of the converged objective, 8k x 8k')
+    ap.add_argument('--no-tune', action='store_true',
+                    help='skip the tile autotuner (use persisted or default tiles)')
     ap.add_argument('--iters', type=int, default=None,
                     help='timecourse / large: override scaling_iter (extra_iter scaled alike) for quick probes')
     args = ap.parse_args()
@@ -497,6 +521,7 @@ def main():
     y = torch.from_numpy(dp.y).cuda()
     g = torch.from_numpy(dp.growth).cuda()
     torch.cuda.synchronize()
+    tiles = _tune_pair(args, dev, x, y, g)
 
     for w in range(args.warmup):
         log('warmup %d' % w)
@@ -545,7 +570,7 @@ def main():
         'dtype': 'f64/f32',
         'data': 'synthetic (seeded Gaussian mixture in 30-d PCA space)',
         'config': {'workload': 'one day pair 15000x15000, d=30, 10 components, fixed_iters, growth_iters=3',
-                   'n_source': n_i, 'n_target': n_j, 'dim': CFG['dim'], **SOLVER},
+                   'n_source': n_i, 'n_target': n_j, 'dim': CFG['dim'], **SOLVER, 'tiles': tiles},
         'sinkhorn_iters_per_s': maps_per_s * iters_per_map,
         'marginal_identity_rel_err': marg_err,
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',

@@ -215,6 +215,41 @@ int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int64_t ldct, 
                     double *grad_x, double *grad_log_g, void *workspace, size_t workspace_bytes,
                     void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Tile shapes of the sweep kernels (no counterpart in the reference, which has
+ * no device kernels).  A tile is (rows of the cost matrix per workgroup,
+ * threads per workgroup).  The library keeps one tile per (kernel, shape
+ * bucket); launches look the tile up by the bucket of their shape.  The
+ * project's autotuner (wot_amd.tune) times every variant on the caller's
+ * workload and stores the fastest here; unset buckets use the built-in
+ * default.  Settings are process-wide and may be changed between calls (not
+ * during one).
+ *   WOTK_KERNEL_ROW_SWEEP:   one stored-cost half-iteration (transport_stablev2,
+ *                            duality-gap solver); bucket from (rows, columns)
+ *                            of the swept matrix.
+ *   WOTK_KERNEL_BATCH_SWEEP: the time-course half-iteration; bucket from the
+ *                            number of combinations and the mean row length
+ *                            sum(2 I J) / sum(I + J) over the pairs.
+ * ------------------------------------------------------------------------- */
+#define WOTK_KERNEL_ROW_SWEEP 0
+#define WOTK_KERNEL_BATCH_SWEEP 1
+
+typedef struct wotk_tile {
+    int32_t rows;    /* cost-matrix rows per workgroup  */
+    int32_t threads; /* threads per workgroup (wave64s) */
+} wotk_tile;
+
+/* number of compiled variants of `kernel`; copies up to `capacity` into out */
+int32_t wotk_tile_variants(int32_t kernel, wotk_tile *out, int32_t capacity);
+/* shape bucket of a launch: ROW_SWEEP (n_rows, n_cols); BATCH_SWEEP
+ * (n_combos, mean row length in n_cols; n_rows ignored).  -1 if kernel unknown */
+int32_t wotk_shape_bucket(int32_t kernel, int64_t n_rows, int64_t n_cols, int32_t n_combos);
+int32_t wotk_num_buckets(int32_t kernel);
+/* tile for one bucket, or every bucket with bucket = -1; tile NULL restores the
+ * default; a tile that is not a compiled variant is rejected (WOTK_EINVAL) */
+int wotk_set_tile(int32_t kernel, int32_t bucket, const wotk_tile *tile);
+int wotk_get_tile(int32_t kernel, int32_t bucket, wotk_tile *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,16 @@ WOTK_EWORKSPACE = 4
 COST_F64 = 0
 COST_BF16X3 = 1
 
+KERNEL_ROW_SWEEP = 0
+KERNEL_BATCH_SWEEP = 1
+
+
+class Tile(ctypes.Structure):
+    _fields_ = [('rows', ctypes.c_int32), ('threads', ctypes.c_int32)]
+
+    def as_tuple(self):
+        return (int(self.rows), int(self.threads))
+
 
 class Stats(ctypes.Structure):
     _fields_ = [('iterations', ctypes.c_int64),
@@ -74,6 +84,11 @@ SIGNATURES = {
                                        _P, _P, _P, _P, _P, _SZ, _P]),
     'wotk_primal_dual': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _P, _P, _D, _D, _D,
                                         ctypes.POINTER(ctypes.c_double), _P, _SZ, _P]),
+    'wotk_tile_variants': (_I32, [_I32, ctypes.POINTER(Tile), _I32]),
+    'wotk_shape_bucket': (_I32, [_I32, _I64, _I64, _I32]),
+    'wotk_num_buckets': (_I32, [_I32]),
+    'wotk_set_tile': (ctypes.c_int, [_I32, _I32, ctypes.POINTER(Tile)]),
+    'wotk_get_tile': (ctypes.c_int, [_I32, _I32, ctypes.POINTER(Tile)]),
 }
 
 

@@ -20,6 +20,8 @@
 // (the terms are then plan entries, bounded by the row mass), with an
 // online-maximum recomputation for any row whose sum under- or overflows.
 #include <algorithm>
+#include <array>
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -27,8 +29,6 @@
 namespace wotk {
 
 constexpr int kMaxCombos = 16;
-constexpr int kBThreads = 256;
-constexpr int kBRows = 2;
 
 struct PairDev {
     long long a_off;   // element offset of this pair's matrix (C or C^T) in the cost buffer
@@ -80,10 +80,11 @@ __device__ __forceinline__ float4 bload4(const float *p) {
     return v;
 }
 
-template <int E>
-__global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
-    constexpr int NW = kBThreads / 64;
-    constexpr int RE = kBRows * E;
+template <int E, int ROWS, int THREADS>
+__global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
+    constexpr int NW = THREADS / 64;
+    constexpr int RE = ROWS * E;
+    static_assert(RE <= THREADS, "one thread per (row, combination) in the epilogue");
     __shared__ int tile_sh;
     __shared__ int bad_sh;
     __shared__ double red_a[RE][NW];
@@ -108,11 +109,11 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
         const float *A = ba.cost + pd.a_off;
         const double *h = ba.other + pd.other_off;
         double *pot = ba.pot + pd.pot_off;
-        const float *rowp[kBRows];
-        double fk[kBRows][E];
-        float s[kBRows][E];
+        const float *rowp[ROWS];
+        double fk[ROWS][E];
+        float s[ROWS][E];
 #pragma unroll
-        for (int r = 0; r < kBRows; ++r) {
+        for (int r = 0; r < ROWS; ++r) {
             const int row = min(tl.row0 + r, pd.nrows - 1);
             rowp[r] = A + (long long)row * pd.lda;
 #pragma unroll
@@ -122,10 +123,10 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
             }
         }
         // main pass: fixed shift f_old * k, one exp2 per (entry, combination)
-        for (int c = tid * 4; c < pd.ncols; c += kBThreads * 4) {
-            double cv[kBRows][4];
+        for (int c = tid * 4; c < pd.ncols; c += THREADS * 4) {
+            double cv[ROWS][4];
 #pragma unroll
-            for (int r = 0; r < kBRows; ++r) {
+            for (int r = 0; r < ROWS; ++r) {
                 const float4 a = bload4(rowp[r] + c);
                 cv[r][0] = a.x;
                 cv[r][1] = a.y;
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
                 const double2 h23 = *reinterpret_cast<const double2 *>(he + 2);
                 const double hk[4] = {h01.x * kk[e], h01.y * kk[e], h23.x * kk[e], h23.y * kk[e]};
 #pragma unroll
-                for (int r = 0; r < kBRows; ++r) {
+                for (int r = 0; r < ROWS; ++r) {
                     float acc = 0.f;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) acc += bexp2((float)fma(-kk[e], cv[r][q], hk[q] + fk[r][e]));
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
             }
         }
 #pragma unroll
-        for (int r = 0; r < kBRows; ++r)
+        for (int r = 0; r < ROWS; ++r)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 float v = s[r][e];  // positive terms: an FP32 tree over 64 lanes loses < 4e-7
@@ -169,17 +170,17 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
         if (bad_sh) {
             // rare: the fixed shift under- or overflowed a row; recompute every
             // (row, combination) of the tile with a per-lane online maximum
-            double sig[kBRows][E];
+            double sig[ROWS][E];
 #pragma unroll
-            for (int r = 0; r < kBRows; ++r)
+            for (int r = 0; r < ROWS; ++r)
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     sig[r][e] = -1.0e6;
                     s[r][e] = 0.f;
                 }
-            for (int c = tid * 4; c < pd.ncols; c += kBThreads * 4) {
+            for (int c = tid * 4; c < pd.ncols; c += THREADS * 4) {
 #pragma unroll
-                for (int r = 0; r < kBRows; ++r) {
+                for (int r = 0; r < ROWS; ++r) {
                     const float4 a = bload4(rowp[r] + c);
                     const double av[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(kBThreads) void batch_sweep_kernel(BatchArgs ba) {
                 }
             }
 #pragma unroll
-            for (int r = 0; r < kBRows; ++r)
+            for (int r = 0; r < ROWS; ++r)
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     double sg = sig[r][e], sd = (double)s[r][e];
@@ -313,8 +314,8 @@ static size_t batch_carve(Arena &ar, const wotk_batch_pair *pairs, int n_pairs, 
         nf += round_up(pairs[p].n_i, 4);
         ng += round_up(pairs[p].n_j, 4);
         nlw += pairs[p].n_i;
-        tf += (pairs[p].n_i + kBRows - 1) / kBRows;
-        tg += (pairs[p].n_j + kBRows - 1) / kBRows;
+        tf += pairs[p].n_i;  // tile lists sized for the smallest tile (one row)
+        tg += pairs[p].n_j;
     }
     w->f = ar.take<double>((size_t)nf * E);
     w->g = ar.take<double>((size_t)ng * E);
@@ -347,49 +348,65 @@ extern "C" size_t wotk_batch_workspace_size(const wotk_batch_pair *pairs, int32_
     return batch_carve(ar, pairs, n_pairs, n_combos, n_iterations, &a, &b, &w, &c, &d);
 }
 
-template <int E>
+// launchers per (combinations, tile variant); variants in the order of kBatchSweepTiles
+typedef int (*SweepFn)(const BatchArgs &, const BatchArgs &, int, int, int *, int *, hipStream_t);
+typedef int (*OccFn)();  // resident workgroups per CU
+
+struct BatchVariant {
+    SweepFn sweep;
+    OccFn occupancy;
+};
+
+template <int E, int V>
 static int launch_sweeps(const BatchArgs &af, const BatchArgs &ag, int grid_f, int grid_g, int *iter, int *queues,
                          hipStream_t s) {
-    batch_sweep_kernel<E><<<grid_f, kBThreads, 0, s>>>(af);
-    batch_sweep_kernel<E><<<grid_g, kBThreads, 0, s>>>(ag);
+    constexpr int R = kBatchSweepTiles[V].rows, T = kBatchSweepTiles[V].threads;
+    batch_sweep_kernel<E, R, T><<<grid_f, T, 0, s>>>(af);
+    batch_sweep_kernel<E, R, T><<<grid_g, T, 0, s>>>(ag);
     batch_advance_kernel<<<1, 1, 0, s>>>(iter, queues);
     WOTK_LAUNCH_CHECK();
     return WOTK_OK;
 }
 
-typedef int (*SweepFn)(const BatchArgs &, const BatchArgs &, int, int, int *, int *, hipStream_t);
+template <int E, int V>
+static int occupancy() {
+    constexpr int R = kBatchSweepTiles[V].rows, T = kBatchSweepTiles[V].threads;
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<E, R, T>, T, 0) != hipSuccess)
+        return 1;
+    return per < 1 ? 1 : per;
+}
 
-static SweepFn pick(int E) {
+template <int E, int... V>
+static constexpr std::array<BatchVariant, sizeof...(V)> variants_of(std::integer_sequence<int, V...>) {
+    return {BatchVariant{launch_sweeps<E, V>, occupancy<E, V>}...};
+}
+
+template <int E>
+static const BatchVariant *variants() {
+    static constexpr auto t = variants_of<E>(std::make_integer_sequence<int, kNumBatchSweepTiles>());
+    return t.data();
+}
+
+static const BatchVariant *pick(int E) {
     switch (E) {
-        case 1: return launch_sweeps<1>;
-        case 2: return launch_sweeps<2>;
-        case 3: return launch_sweeps<3>;
-        case 4: return launch_sweeps<4>;
-        case 6: return launch_sweeps<6>;
-        case 9: return launch_sweeps<9>;
-        case 12: return launch_sweeps<12>;
-        case 16: return launch_sweeps<16>;
+        case 1: return variants<1>();
+        case 2: return variants<2>();
+        case 3: return variants<3>();
+        case 4: return variants<4>();
+        case 6: return variants<6>();
+        case 9: return variants<9>();
+        case 12: return variants<12>();
+        case 16: return variants<16>();
         default: return nullptr;
     }
 }
 
-static int persistent_grid(int E) {
-    int dev = 0, ncu = 0, per = 0;
+static int persistent_grid(const BatchVariant &v) {
+    int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 1024;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
-    hipError_t e = hipSuccess;
-    switch (E) {
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<1>, kBThreads, 0); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<2>, kBThreads, 0); break;
-        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<3>, kBThreads, 0); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<4>, kBThreads, 0); break;
-        case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<6>, kBThreads, 0); break;
-        case 9: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<9>, kBThreads, 0); break;
-        case 12: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<12>, kBThreads, 0); break;
-        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_sweep_kernel<16>, kBThreads, 0); break;
-    }
-    if (e != hipSuccess || per < 1) per = 1;
-    return ncu * per;
+    return ncu * v.occupancy();
 }
 
 extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pairs, int32_t n_pairs, const double *G,
@@ -418,6 +435,16 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     batch_carve(ar, pairs, n_pairs, E, n_iter, &nf, &ng, &w, &ntf, &ntg);
     if (!workspace || !ar.ok) return fail(WOTK_EWORKSPACE, "batch workspace too small");
 
+    // tile variant from the shape bucket: combinations x mean row length over both sides
+    double entries = 0.0, rows_total = 0.0;
+    for (int p = 0; p < n_pairs; ++p) {
+        entries += 2.0 * (double)pairs[p].n_i * (double)pairs[p].n_j;
+        rows_total += (double)pairs[p].n_i + (double)pairs[p].n_j;
+    }
+    const int vi = tile_for(WOTK_KERNEL_BATCH_SWEEP, 0, (int64_t)(entries / rows_total), E);
+    const BatchVariant &variant = pick(E)[vi];
+    const int tile_rows = kBatchSweepTiles[vi].rows;
+
     // host-side tables: pairs, tiles (longest rows first), iteration parameters
     std::vector<PairDev> pf(n_pairs), pg(n_pairs);
     std::vector<long long> goff(n_pairs), lwoff(n_pairs);
@@ -441,7 +468,7 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     auto tiles_for = [&](const std::vector<PairDev> &pd) {
         std::vector<Tile> t;
         for (int p = 0; p < n_pairs; ++p)
-            for (int r = 0; r < pd[p].nrows; r += kBRows) t.push_back(Tile{p, r});
+            for (int r = 0; r < pd[p].nrows; r += tile_rows) t.push_back(Tile{p, r});
         std::stable_sort(t.begin(), t.end(), [&](const Tile &a, const Tile &b) {
             return pd[a.side_pair].ncols > pd[b.side_pair].ncols;
         });
@@ -479,11 +506,12 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     WOTK_LAUNCH_CHECK();
     WOTK_HIP(hipStreamSynchronize(s));  // host tables above are released at return
 
-    BatchArgs af{cost, w.pairs_f, w.tiles_f, (int)ntf, w.table, w.iter, E, 0, w.f, w.g, w.logw, w.logq, w.lognf, w.queues};
-    BatchArgs ag{cost, w.pairs_g, w.tiles_g, (int)ntg, w.table, w.iter, E, 1, w.g, w.f, w.logw, w.logq, w.logng, w.queues + 1};
-    const int grid = persistent_grid(E);
-    const int grid_f = (int)std::min<int64_t>(grid, ntf), grid_g = (int)std::min<int64_t>(grid, ntg);
-    SweepFn sweep = pick(E);
+    const int64_t n_tf = (int64_t)tf.size(), n_tg = (int64_t)tg.size();  // <= ntf, ntg (one-row tiles)
+    BatchArgs af{cost, w.pairs_f, w.tiles_f, (int)n_tf, w.table, w.iter, E, 0, w.f, w.g, w.logw, w.logq, w.lognf, w.queues};
+    BatchArgs ag{cost, w.pairs_g, w.tiles_g, (int)n_tg, w.table, w.iter, E, 1, w.g, w.f, w.logw, w.logq, w.logng, w.queues + 1};
+    const int grid = persistent_grid(variant);
+    const int grid_f = (int)std::min<int64_t>(grid, n_tf), grid_g = (int)std::min<int64_t>(grid, n_tg);
+    const SweepFn sweep = variant.sweep;
 
     hipEvent_t ev[2];
     WOTK_HIP(hipEventCreate(&ev[0]));

@@ -39,6 +39,21 @@ struct Arena {
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+// Compiled tile variants of the sweep kernels, (rows per workgroup, threads).
+// Entry 0 of each list is the built-in default.  The kernels are instantiated
+// from these lists, so a variant exists exactly when it is listed here.
+struct TileShape {
+    int rows, threads;
+};
+constexpr TileShape kRowSweepTiles[] = {{4, 256}, {1, 256}, {2, 256}, {8, 256},
+                                        {2, 512}, {4, 512}, {8, 512}, {4, 128}};
+constexpr int kNumRowSweepTiles = sizeof(kRowSweepTiles) / sizeof(kRowSweepTiles[0]);
+constexpr TileShape kBatchSweepTiles[] = {{2, 256}, {1, 256}, {4, 256}, {1, 512}, {2, 512}};
+constexpr int kNumBatchSweepTiles = sizeof(kBatchSweepTiles) / sizeof(kBatchSweepTiles[0]);
+
+// variant index (into the lists above) a launch of this shape uses
+int tile_for(int kernel, int64_t n_rows, int64_t n_cols, int n_combos);
+
 }  // namespace wotk
 
 #define WOTK_HIP(expr)                                                                       \

@@ -16,6 +16,9 @@
 // exactly (ldexp) in FP64.  Absorption events are tracked only where the
 // reference's control flow observes them (the duality-gap solver's
 // dual-evolution check).
+#include <array>
+#include <utility>
+
 #include "common.h"
 
 namespace wotk {
@@ -76,8 +79,8 @@ struct RowEpilogue {
     double *lse_out;         // optional
 };
 
-template <int ROWS>
-__global__ __launch_bounds__(kThreads) void lse_rows_kernel(const float *__restrict__ A, int64_t lda,
+template <int ROWS, int THREADS>
+__global__ __launch_bounds__(THREADS) void lse_rows_kernel(const float *__restrict__ A, int64_t lda,
                                                             int nrows, int ncols,
                                                             const double *__restrict__ h, double k,
                                                             RowEpilogue ep) {
@@ -92,7 +95,7 @@ __global__ __launch_bounds__(kThreads) void lse_rows_kernel(const float *__restr
         sig[r] = kSigInit;
         s[r] = 0.f;
     }
-    for (int c = tid * 4; c < ncols; c += kThreads * 4) {
+    for (int c = tid * 4; c < ncols; c += THREADS * 4) {
         const double2 h01 = *reinterpret_cast<const double2 *>(h + c);
         const double2 h23 = *reinterpret_cast<const double2 *>(h + c + 2);
         float4 a[ROWS];
@@ -118,8 +121,8 @@ __global__ __launch_bounds__(kThreads) void lse_rows_kernel(const float *__restr
             s[r] += (e0 + e1) + (e2 + e3);
         }
     }
-    __shared__ double lsig[ROWS][kThreads / 64];
-    __shared__ double lsum[ROWS][kThreads / 64];
+    __shared__ double lsig[ROWS][THREADS / 64];
+    __shared__ double lsum[ROWS][THREADS / 64];
     const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(kThreads) void lse_rows_kernel(const float *__restr
         const int row = r0 + tid;
         double sg = lsig[tid][0], sd = lsum[tid][0];
 #pragma unroll
-        for (int w = 1; w < kThreads / 64; ++w) lse_merge(sg, sd, lsig[tid][w], lsum[tid][w]);
+        for (int w = 1; w < THREADS / 64; ++w) lse_merge(sg, sd, lsig[tid][w], lsum[tid][w]);
         if (row < nrows) {
             const double lse = (sg + log2(sd)) * kLn2;
             const double lw = ep.logw ? ep.logw[row] : *ep.logw_dev;
@@ -491,11 +494,28 @@ static int init_state(const Problem &pb, Work &w, hipStream_t s) {
     return WOTK_OK;
 }
 
+// one launcher per compiled tile variant, in the order of kRowSweepTiles
+typedef void (*LseLaunch)(const float *, int64_t, int, int, const double *, double, const RowEpilogue &,
+                          hipStream_t);
+
+template <int V>
+static void lse_variant(const float *A, int64_t lda, int nrows, int ncols, const double *h, double k,
+                        const RowEpilogue &ep, hipStream_t s) {
+    constexpr int R = kRowSweepTiles[V].rows, T = kRowSweepTiles[V].threads;
+    lse_rows_kernel<R, T><<<(nrows + R - 1) / R, T, 0, s>>>(A, lda, nrows, ncols, h, k, ep);
+}
+
+template <int... V>
+static constexpr std::array<LseLaunch, sizeof...(V)> lse_table(std::integer_sequence<int, V...>) {
+    return {lse_variant<V>...};
+}
+
+static constexpr auto kLseLaunch = lse_table(std::make_integer_sequence<int, kNumRowSweepTiles>());
+
 static int launch_lse(const float *A, int64_t lda, int nrows, int ncols, const double *h, double eps,
                       const RowEpilogue &ep, hipStream_t s) {
     const double k = kLog2E / eps;
-    const int nb = (nrows + kRows - 1) / kRows;
-    lse_rows_kernel<kRows><<<nb, kThreads, 0, s>>>(A, lda, nrows, ncols, h, k, ep);
+    kLseLaunch[tile_for(WOTK_KERNEL_ROW_SWEEP, nrows, ncols, 1)](A, lda, nrows, ncols, h, k, ep, s);
     WOTK_LAUNCH_CHECK();
     return WOTK_OK;
 }

@@ -92,6 +92,42 @@ def build_costs(coords, pairs_idx, cost_mode='f64', device='cuda'):
     return buf, out, [(l[4], l[5]) for l in layout]
 
 
+class TimeCourseCosts:
+    """Device-resident inputs of a batched time course: every pair's FP32
+    (C, C^T) in one buffer, the pair table and the concatenated growth."""
+
+    def __init__(self, buf, bps, shapes, G, pairs):
+        self.buf = buf
+        self.bps = bps
+        self.shapes = shapes
+        self.G = G
+        self.pairs = pairs
+        self.arr = (_native.BatchPair * len(bps))(*bps)
+
+    @property
+    def n_pairs(self):
+        return len(self.bps)
+
+    def mean_row_length(self):
+        """sum(2 I J) / sum(I + J): the row length the tile table is keyed on."""
+        e = sum(2.0 * i * j for i, j in self.shapes)
+        r = sum(float(i + j) for i, j in self.shapes)
+        return int(e / r)
+
+
+def prepare_time_course(coords, growth, pairs=None, cost_mode='f64', device='cuda'):
+    """Build the costs of every consecutive day pair (or the given index pairs)."""
+    torch = _dev.torch_mod()
+    if pairs is None:
+        pairs = [(t, t + 1) for t in range(len(coords) - 1)]
+    buf, bps, shapes = build_costs(coords, pairs, cost_mode=cost_mode, device=device)
+    goffs = np.cumsum([0] + [shapes[k][0] for k in range(len(pairs))])
+    G = torch.cat([_dev.as_device_f64(torch, growth[a], device).view(-1) for (a, _) in pairs])
+    for k, bp in enumerate(bps):
+        bp.g_offset = int(goffs[k])
+    return TimeCourseCosts(buf, bps, shapes, G, pairs)
+
+
 def solve_time_course(coords, growth, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_iter=3000,
                       tau=1e4, epsilon0=1.0, extra_iter=1000, inner_iter_max=50, graph_iters=50,
                       cost_mode='f64', pairs=None, group_size=None, device='cuda'):
@@ -100,17 +136,22 @@ def solve_time_course(coords, growth, epsilons=(0.05,), lambda1s=(1.0,), lambda2
     PCA coordinates of day t, ``growth[t]`` its cells' growth (G = growth, dt=1).
     ``group_size`` splits the grid into batched runs of that many combinations
     (each cost entry is then read once per group instead of once overall)."""
+    tc = prepare_time_course(coords, growth, pairs=pairs, cost_mode=cost_mode, device=device)
+    return solve_prepared(tc, epsilons, lambda1s, lambda2=lambda2, scaling_iter=scaling_iter, tau=tau,
+                          epsilon0=epsilon0, extra_iter=extra_iter, inner_iter_max=inner_iter_max,
+                          graph_iters=graph_iters, group_size=group_size)
+
+
+def solve_prepared(tc, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_iter=3000, tau=1e4, epsilon0=1.0,
+                   extra_iter=1000, inner_iter_max=50, graph_iters=50, group_size=None):
+    """``solve_time_course`` on costs already built by ``prepare_time_course``."""
     torch = _dev.torch_mod()
-    if pairs is None:
-        pairs = [(t, t + 1) for t in range(len(coords) - 1)]
+    from .. import tune
+    tune.apply_persisted()
+    device = tc.buf.device
+    buf, bps, shapes, G, arr = tc.buf, tc.bps, tc.shapes, tc.G, tc.arr
     combos = list(itertools.product(epsilons, lambda1s))
     E = len(combos)
-    buf, bps, shapes = build_costs(coords, pairs, cost_mode=cost_mode, device=device)
-    goffs = np.cumsum([0] + [shapes[k][0] for k in range(len(pairs))])
-    G = torch.cat([_dev.as_device_f64(torch, growth[a], device).view(-1) for (a, _) in pairs])
-    for k, bp in enumerate(bps):
-        bp.g_offset = int(goffs[k])
-    arr = (_native.BatchPair * len(bps))(*bps)
     n_iter = scaling_iter + extra_iter
     L = _native.lib()
     gsz = E if group_size is None else int(group_size)

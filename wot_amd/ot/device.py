@@ -150,6 +150,8 @@ def solve(cost, G, solver, params, want_plan=True, want_row_sums=False, want_pot
     if Gd.numel() != n_i:
         raise ValueError('G has %d entries for %d source cells' % (Gd.numel(), n_i))
     L = _native.lib()
+    from .. import tune
+    tune.apply_persisted()
     ws = workspace(torch, L.wotk_solver_workspace_size(n_i, n_j), dev)
     plan = torch.empty((n_i, n_j), dtype=torch.float64, device=dev) if want_plan else None
     rows = torch.empty(n_i, dtype=torch.float64, device=dev) if want_row_sums else None
