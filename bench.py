@@ -5,7 +5,10 @@ FP64 CPU oracle.
 Workload (one "step" = one transport map, as OTModel.compute_transport_map
 computes it): one day pair of 15,000 x 15,000 cells, 30-d PCA coordinates from
 a seeded 10-component Gaussian mixture (target means shifted by N(0, 0.5)),
-log-normal growth; cost = squared Euclidean / median (FP64 on the device);
+log-normal growth; cost = squared Euclidean / median on the device, cross term
+as a 6-term split-BF16 MFMA GEMM with FP32 accumulation and FP64 norms
+(``--cost-mode``; the 3-term split misses the 1e-5 plan gate, see the
+``cost_split_bf16`` report);
 fixed_iters solver (eps=0.05, lambda1=1, lambda2=50, eps0=1, tau=1e4,
 scaling_iter=3000, inner_iter_max=50, extra_iter=1000) with growth_iters=3,
 i.e. 12,000 Sinkhorn iterations per map; the FP64 plan is materialised in HBM.
@@ -44,9 +47,9 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def one_map(ot, dev, x, y, growth):
+def one_map(ot, dev, x, y, growth, cost_mode='bf16x6'):
     """cost + median on the device, growth loop, final FP64 plan in HBM."""
-    cost = dev.DeviceCost.from_coordinates(x, y)
+    cost = dev.DeviceCost.from_coordinates(x, y, mode=cost_mode)
     params = dict(SOLVER, C=cost, G=growth, device_out=True)
     sweep_ms, sweeps = 0.0, 0
     G = params['G']
@@ -61,6 +64,56 @@ def one_map(ot, dev, x, y, growth):
         sweeps += res['stats']['sweeps']
     res['G_last'] = G
     return cost, res, sweep_ms, sweeps
+
+
+def cost_gemm_report(torch, x, y, G=None, solve=None):
+    """Split-BF16 MFMA costs (3 and 6 terms) against the FP64 direct cost on
+    one pair: error of the normalised FP32 cost (max, RMS, RMS over the nearest
+    1 %% of pairs), median error, cost-build wall times, and, if ``solve``
+    (fixed_iters parameters) is given, the relative Frobenius error of the plan
+    computed from each split cost against the plan from the FP64 cost."""
+    import wot_amd.ot as ot
+    from wot_amd.ot import device as dev
+    out = {'shape': [int(x.shape[0]), int(y.shape[0]), int(x.shape[1])]}
+    costs = {}
+    for mode in ('f64', 'bf16x3', 'bf16x6'):
+        dev.DeviceCost.from_coordinates(x, y, mode=mode)  # warm-up (code objects, allocator)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        costs[mode] = dev.DeviceCost.from_coordinates(x, y, mode=mode)
+        torch.cuda.synchronize()
+        out['build_s_' + mode] = time.perf_counter() - t0
+    cf = costs['f64'].dense()
+    n_i, n_j = cf.shape
+    # 1 %% quantile of the cost from a fixed sample of entries: "nearest pairs" threshold
+    gen = torch.Generator(device=cf.device).manual_seed(0)
+    idx = torch.randint(0, n_i * n_j, (min(n_i * n_j, 1 << 20),), device=cf.device, generator=gen)
+    thr = torch.quantile(cf[idx // n_j, idx % n_j].double(), 0.01)
+    step = max(1, (1 << 27) // n_j)  # row chunks of ~128 M entries
+    Rf = ot.transport_stablev2(costs['f64'], G=G, device_out=True, **solve) if solve is not None else None
+    for mode in ('bf16x3', 'bf16x6'):
+        cb = costs[mode].dense()
+        mx, ss, ss_near, n_near = 0.0, 0.0, 0.0, 0
+        for r0 in range(0, n_i, step):
+            a = cf[r0:r0 + step].double()
+            d = cb[r0:r0 + step].double() - a
+            mx = max(mx, float(d.abs().max()))
+            ss += float(d.pow(2).sum())
+            m = a <= thr
+            ss_near += float(d[m].pow(2).sum())
+            n_near += int(m.sum())
+            del a, d, m
+        r = {'max_abs_err': mx, 'rms_err': (ss / (n_i * n_j)) ** 0.5,
+             'rms_err_nearest_1pct': (ss_near / max(n_near, 1)) ** 0.5,
+             'median_rel_err': float((costs[mode].median - costs['f64'].median).abs() / costs['f64'].median)}
+        if Rf is not None:
+            Rb = ot.transport_stablev2(costs[mode], G=G, device_out=True, **solve)
+            r['plan_rel_frob_vs_f64_cost'] = float(torch.linalg.norm(Rb - Rf) / torch.linalg.norm(Rf))
+            del Rb
+        out[mode] = r
+    del costs, cf, Rf
+    torch.cuda.empty_cache()
+    return out
 
 
 def _tune_pair(args, dev, x, y, g):
@@ -211,6 +264,9 @@ def bench_timecourse(args):
             worst = max(worst, float(torch.linalg.norm(R1 - R2) / torch.linalg.norm(R1)))
             del R1, R2
     out['parity_vs_single_solver_rel_frob'] = worst
+    order = np.argsort(pair_ij)
+    out['cost_split_bf16'] = [cost_gemm_report(torch, coords[p], coords[p + 1])
+                          for p in (int(order[0]), int(order[len(order) // 2]), int(order[-1]))]
     if not args.no_baselines:
         out.update(_timecourse_baselines(torch, coords, growth, pair_ij, n_combo, sched, n_iter, elapsed))
     return out
@@ -326,6 +382,7 @@ def bench_adjoint(args):
            'config': {'workload': 'd OT / d(x, log g), 8000 x 8000, d=30, duality_gap solver'},
            'forward_s': fwd, 'adjoint_s': bwd, 'objective': float(val)}
     out.update(_adjoint_checks(torch, ot, dp, x, lg, y))
+    out['cost_split_bf16'] = cost_gemm_report(torch, dp.x, dp.y)
     return out
 
 
@@ -454,6 +511,7 @@ def bench_large(args):
                                                                                 torch.linalg.norm(Rref))
     del cost, D, C, R, Rref
     torch.cuda.empty_cache()
+    out['cost_split_bf16_stored_kernel'] = cost_gemm_report(torch, X, Y)
     # 5-map chain: 6 days x 12,000 cells solved with the stored-cost solver
     coords, growth, counts = time_course(dim=50, n_components=30, seed=7, counts=[12000] * 6)
     maps = []
@@ -491,6 +549,8 @@ def main():
     ap.add_argument('--config', default='pair', choices=['pair', 'timecourse', 'adjoint', 'large'],
                     help='pair: one 15k x 15k map (default); timecourse: 38 pairs x 9-point grid; '
                          'adjoint: gradient of the converged objective, 8k x 8k')
+    ap.add_argument('--cost-mode', default='bf16x6', choices=['f64', 'bf16x3', 'bf16x6'],
+                    help='pair only: cost kernel of the timed maps')
     ap.add_argument('--no-tune', action='store_true',
                     help='skip the tile autotuner (use persisted or default tiles)')
     ap.add_argument('--iters', type=int, default=None,
@@ -525,14 +585,14 @@ def main():
 
     for w in range(args.warmup):
         log('warmup %d' % w)
-        one_map(ot, dev, x, y, g)
+        one_map(ot, dev, x, y, g, args.cost_mode)
     torch.cuda.synchronize()
     sweep_ms = 0.0
     sweeps = 0
     t0 = time.perf_counter()
     res = cost = None
     for s in range(args.steps):
-        cost, res, sms, nsw = one_map(ot, dev, x, y, g)
+        cost, res, sms, nsw = one_map(ot, dev, x, y, g, args.cost_mode)
         sweep_ms += sms
         sweeps += nsw
         log('step %d done' % s)
@@ -570,7 +630,8 @@ def main():
         'dtype': 'f64/f32',
         'data': 'synthetic (seeded Gaussian mixture in 30-d PCA space)',
         'config': {'workload': 'one day pair 15000x15000, d=30, 10 components, fixed_iters, growth_iters=3',
-                   'n_source': n_i, 'n_target': n_j, 'dim': CFG['dim'], **SOLVER, 'tiles': tiles},
+                   'n_source': n_i, 'n_target': n_j, 'dim': CFG['dim'], **SOLVER, 'cost_mode': args.cost_mode,
+                   'tiles': tiles},
         'sinkhorn_iters_per_s': maps_per_s * iters_per_map,
         'marginal_identity_rel_err': marg_err,
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
@@ -602,6 +663,9 @@ def main():
                                          'extrapolated to 3 x (4000 iterations + 61 builds)',
                                **cpu_detail}
         out['speedup_vs_cpu'] = maps_per_s * per_map_cpu
+    log('split-BF16 cost report')
+    solve = {k: v for k, v in SOLVER.items() if k != 'growth_iters'}
+    out['cost_split_bf16'] = cost_gemm_report(torch, x, y, G=g, solve=solve)
     print(json.dumps(out), flush=True)
 
 

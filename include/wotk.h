@@ -42,6 +42,7 @@ extern "C" {
 /* cost modes for wotk_cost_sqeuclid */
 #define WOTK_COST_F64 0       /* FP64 direct difference sum_k (x_k - y_k)^2     */
 #define WOTK_COST_BF16X3 1    /* split-BF16 3-term MFMA cross term, FP32 accum  */
+#define WOTK_COST_BF16X6 2    /* split-BF16 6-term MFMA cross term, FP32 accum  */
 
 typedef struct wotk_stats {
     int64_t iterations;       /* Sinkhorn (a, b) iterations performed            */
@@ -68,9 +69,11 @@ int wotk_abi_version(void);
  * FP64 median to *median_out (device scalar) and, if median_index_out is not
  * NULL, the flat indices (i*J + j) of the middle element(s) as in
  * wotk_median_f64.  mode = WOTK_COST_F64 computes
- * the FP64 direct difference (scipy's formula); WOTK_COST_BF16X3 computes
- * ||x||^2 + ||y||^2 - 2 x.y with the cross term as a 3-term split-BF16 MFMA
- * GEMM accumulated in FP32.  The median is exact (radix select over the FP64
+ * the FP64 direct difference (scipy's formula); WOTK_COST_BF16X3 / _BF16X6
+ * compute ||x - m||^2 + ||y - m||^2 - 2 (x - m).(y - m) (m = mean of both
+ * days) with the cross term as a 3- or 6-term split-BF16 MFMA GEMM
+ * accumulated in FP32 (x = hi + mid + lo in BF16; 3 terms hi.hi + hi.mid +
+ * mid.hi, 6 terms add hi.lo + lo.hi + mid.mid).  The median is exact (radix select over the FP64
  * bit patterns; even count -> mean of the two middle values, numpy.median).
  * ------------------------------------------------------------------------- */
 size_t wotk_cost_workspace_size(int64_t I, int64_t J, int64_t d, int32_t mode);

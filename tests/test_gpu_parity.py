@@ -202,3 +202,27 @@ def test_cost_bf16x3_error_is_bounded(ot):
                                                                   np.sqrt((err[near] ** 2).mean())))
     assert err.max() < 5e-5 and np.sqrt((err ** 2).mean()) < 5e-6
     assert np.array_equal(approx.CT[:, :2000].cpu().numpy(), approx.dense().cpu().numpy().T)
+
+
+def test_cost_bf16x6_error_and_plan_parity(ot):
+    """Split-BF16 (6-term) MFMA cost: cross-term error ~2^-24 relative, i.e. at
+    the FP32 rounding of the stored cost, and the plan solved on it meets the
+    1e-5 gate against the FP64 oracle (the 3-term cost does not, at eps=0.05)."""
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(2000, 2500, dim=30, seed=0)
+    exact = ot.DeviceCost.from_coordinates(dp.x, dp.y)
+    approx = ot.DeviceCost.from_coordinates(dp.x, dp.y, mode='bf16x6')
+    a = approx.dense().cpu().numpy().astype(np.float64)
+    e = exact.dense().cpu().numpy().astype(np.float64)
+    err = np.abs(a - e)
+    print('bf16x6 cost error: max %.3e rms %.3e' % (err.max(), np.sqrt((err ** 2).mean())))
+    assert err.max() < 2e-6 and np.sqrt((err ** 2).mean()) < 2e-7
+    C = O.default_cost_matrix(dp.x, dp.y)
+    kw = dict(lambda1=1.0, lambda2=50.0, epsilon=0.05, scaling_iter=300, tau=1e4, epsilon0=1.0, extra_iter=100,
+              inner_iter_max=50)
+    Rref = O.transport_stablev2(C, G=dp.growth, **kw)
+    R = ot.transport_stablev2(approx, G=dp.growth, **kw)
+    rel = np.linalg.norm(R - Rref) / np.linalg.norm(Rref)
+    marg = max(np.abs(R.sum(ax) - Rref.sum(ax)).max() / np.abs(Rref.sum(ax)).max() for ax in (0, 1))
+    print('bf16x6 plan rel frob %.3e marginals %.3e' % (rel, marg))
+    assert rel <= PLAN_TOL and marg <= MARG_TOL

@@ -17,6 +17,8 @@ WOTK_EWORKSPACE = 4
 
 COST_F64 = 0
 COST_BF16X3 = 1
+COST_BF16X6 = 2
+COST_MODES = {'f64': COST_F64, 'bf16x3': COST_BF16X3, 'bf16x6': COST_BF16X6}
 
 KERNEL_ROW_SWEEP = 0
 KERNEL_BATCH_SWEEP = 1

@@ -293,9 +293,9 @@ int launch_sqeuclid_f64(const double *X, const double *Y, int64_t I, int64_t J, 
     return WOTK_OK;
 }
 
-int launch_cost_bf16x3(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
-                       double *C64, int64_t ld, void *scratch, hipStream_t s);
-size_t cost_bf16x3_scratch(int64_t I, int64_t J, int64_t d);
+int launch_cost_bf16(const double *X, const double *Y, int64_t I, int64_t J, int64_t d, int terms,
+                     double *C64, int64_t ld, void *scratch, hipStream_t s);
+size_t cost_bf16_scratch(int64_t I, int64_t J, int64_t d);
 
 static int check_cost_layout(int64_t I, int64_t J, int64_t d, const float *C, int64_t ldc,
                              const float *CT, int64_t ldct) {
@@ -336,7 +336,7 @@ extern "C" size_t wotk_cost_workspace_size(int64_t I, int64_t J, int64_t d, int3
     SelectState *st;
     unsigned long long *h;
     median_ws(ar, I * J, &st, &h);
-    if (mode == WOTK_COST_BF16X3) ar.take<char>(cost_bf16x3_scratch(I, J, d));
+    if (mode == WOTK_COST_BF16X3 || mode == WOTK_COST_BF16X6) ar.take<char>(cost_bf16_scratch(I, J, d));
     return ar.off;
 }
 
@@ -353,7 +353,8 @@ extern "C" int wotk_cost_sqeuclid(const double *X, const double *Y, int64_t I, i
                                   size_t workspace_bytes, void *stream) {
     int rc = check_cost_layout(I, J, d, C, ldc, CT, ldct);
     if (rc) return rc;
-    WOTK_REQUIRE(mode == WOTK_COST_F64 || mode == WOTK_COST_BF16X3, "unknown cost mode %d", mode);
+    WOTK_REQUIRE(mode == WOTK_COST_F64 || mode == WOTK_COST_BF16X3 || mode == WOTK_COST_BF16X6,
+                 "unknown cost mode %d", mode);
     hipStream_t s = (hipStream_t)stream;
     Arena ar(workspace, workspace_bytes);
     double *C64 = ar.take<double>((size_t)I * J);
@@ -361,10 +362,11 @@ extern "C" int wotk_cost_sqeuclid(const double *X, const double *Y, int64_t I, i
     unsigned long long *h;
     median_ws(ar, I * J, &st, &h);
     void *scratch = nullptr;
-    if (mode == WOTK_COST_BF16X3) scratch = ar.take<char>(cost_bf16x3_scratch(I, J, d));
+    if (mode != WOTK_COST_F64) scratch = ar.take<char>(cost_bf16_scratch(I, J, d));
     if (!ar.ok || !workspace) return fail(WOTK_EWORKSPACE, "cost workspace too small");
     rc = (mode == WOTK_COST_F64) ? launch_sqeuclid_f64(X, Y, I, J, d, C64, J, s)
-                                 : launch_cost_bf16x3(X, Y, I, J, d, C64, J, scratch, s);
+                                 : launch_cost_bf16(X, Y, I, J, d, mode == WOTK_COST_BF16X3 ? 3 : 6, C64, J,
+                                                    scratch, s);
     if (rc) return rc;
     rc = median_run(C64, I * J, nullptr, median_index_out, st, h, s);
     if (rc) return rc;

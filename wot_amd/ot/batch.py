@@ -68,7 +68,7 @@ def build_costs(coords, pairs_idx, cost_mode='f64', device='cuda'):
     """All pairs' FP32 (C, C^T) in one device buffer; returns (buffer, BatchPair list, shapes)."""
     torch = _dev.torch_mod()
     L = _native.lib()
-    mode = {'f64': _native.COST_F64, 'bf16x3': _native.COST_BF16X3}[cost_mode]
+    mode = _native.COST_MODES[cost_mode]
     layout = []
     off = 0
     for (a, b) in pairs_idx:

@@ -79,7 +79,7 @@ class DeviceCost:
             raise ValueError('coordinates must be 2-D with the same number of columns')
         n_i, d = X.shape
         n_j = Y.shape[0]
-        m = {'f64': _native.COST_F64, 'bf16x3': _native.COST_BF16X3}[mode]
+        m = _native.COST_MODES[mode]
         L = _native.lib()
         ws = workspace(torch, L.wotk_cost_workspace_size(n_i, n_j, d, m), device)
         C, CT = cls._alloc(torch, n_i, n_j, device)

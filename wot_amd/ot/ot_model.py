@@ -38,7 +38,7 @@ class OTModel:
     ``obs[day_field]``, optional ``obs[growth_rate_field]``), and OT
     configuration keyword arguments (``epsilon``, ``lambda1``, ``lambda2``,
     ``growth_iters``, ``local_pca``, ``solver`` = 'duality_gap' (default) or
-    'fixed_iters', ...).  ``cost_mode`` ('f64' default, or 'bf16x3') selects
+    'fixed_iters', ...).  ``cost_mode`` ('f64' default, 'bf16x3' or 'bf16x6') selects
     the device cost kernel."""
 
     def __init__(self, matrix, day_field='day', covariate_field='covariate', growth_rate_field='cell_growth_rate',
