@@ -147,7 +147,18 @@ int wotk_plan(const float *C, int64_t ldc, int64_t I, int64_t J, const double *f
  * (device, may be NULL) receive the final potentials as consecutive blocks
  * [pair][combination][I_p] (resp. [J_p]).  graph_iters > 0 captures that many
  * iterations in a hipGraph once and replays it.
+ * flags & WOTK_BATCH_SHARE_EXP: combinations are taken in consecutive groups
+ * of equal epsilon (n_combos = groups x group size, e.g. an epsilon x lambda1
+ * grid in epsilon-major order); within a group one exponential per cost entry
+ * serves every combination (exp2(y_ref) times a per-column factor), tiles
+ * whose factors leave a safe range are recomputed exactly.  Ignored (the
+ * per-combination kernel runs) when the grid has no such grouping.
+ * stats: sweeps / sweep_ms as for the single solver; stabilisations = tiles
+ * recomputed exactly (fixed shift out of range); gap_checks = combinations per
+ * shared-exponential group (0: one exponential per combination).
  * ------------------------------------------------------------------------- */
+#define WOTK_BATCH_SHARE_EXP 1
+
 typedef struct wotk_batch_pair {
     int64_t c_offset;   /* element offset of C_p (I_p x J_p, ldc) in the cost buffer */
     int64_t ct_offset;  /* element offset of C_p^T (J_p x I_p, ldct)                 */
@@ -162,7 +173,7 @@ int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pairs, int32_t
                         const double *G, int32_t n_combos, const double *epsilon,
                         const double *lambda1, double lambda2, int64_t scaling_iter, double tau,
                         double epsilon0, int64_t extra_iter, int64_t inner_iter_max, double *f_out,
-                        double *g_out, int32_t graph_iters, void *workspace,
+                        double *g_out, int32_t graph_iters, int32_t flags, void *workspace,
                         size_t workspace_bytes, wotk_stats *stats, void *stream);
 
 /* ---------------------------------------------------------------------------
@@ -233,9 +244,11 @@ int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int64_t ldct, 
  *   WOTK_KERNEL_BATCH_SWEEP: the time-course half-iteration; bucket from the
  *                            number of combinations and the mean row length
  *                            sum(2 I J) / sum(I + J) over the pairs.
+ *   WOTK_KERNEL_BATCH_SHARED: the same with WOTK_BATCH_SHARE_EXP; same buckets.
  * ------------------------------------------------------------------------- */
 #define WOTK_KERNEL_ROW_SWEEP 0
 #define WOTK_KERNEL_BATCH_SWEEP 1
+#define WOTK_KERNEL_BATCH_SHARED 2
 
 typedef struct wotk_tile {
     int32_t rows;    /* cost-matrix rows per workgroup  */

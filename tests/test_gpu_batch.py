@@ -38,13 +38,43 @@ def _check(res, coords, growth, epsilon0, lambda2=50.0, tol=1e-5):
     return worst
 
 
-def test_batch_grid_matches_oracle(batch):
+@pytest.mark.parametrize('share_exp', [True, False])
+def test_batch_grid_matches_oracle(batch, share_exp):
     coords, growth = _course(1, [150, 320, 200, 410])
     res = batch.solve_time_course(coords, growth, epsilons=(0.01, 0.05, 0.1), lambda1s=(0.5, 1.0, 5.0),
-                                  graph_iters=50, **SCHED)
+                                  graph_iters=50, share_exp=share_exp, **SCHED)
     assert len(res.combos) == 9
+    assert res.stats['shared_group'] == (3 if share_exp else 0)
     worst = _check(res, coords, growth, 1.0)
-    print('batch worst rel frob %.3e' % worst)
+    print('batch worst rel frob %.3e (share_exp=%s, exact tiles %d)' % (worst, share_exp, res.stats['exact_tiles']))
+
+
+@pytest.mark.parametrize('epsilons,lambda1s,group', [
+    ((0.05, 0.1), (1.0, 5.0), 2),          # 2 groups of 2
+    ((0.05, 0.1), (0.5, 1.0, 5.0), 3),     # 2 groups of 3
+    ((0.05,), (0.5, 5.0), 2),              # 1 group of 2
+    ((0.01, 0.05, 0.1), (1.0,), 0),        # no equal epsilons: per-combination kernel
+])
+def test_shared_exponential_group_shapes(batch, epsilons, lambda1s, group):
+    coords, growth = _course(7, [97, 210, 160])
+    res = batch.solve_time_course(coords, growth, epsilons=epsilons, lambda1s=lambda1s, graph_iters=50, **SCHED)
+    assert res.stats['shared_group'] == group
+    _check(res, coords, growth, 1.0)
+
+
+def test_shared_exponential_far_potentials_fall_back(batch):
+    """eps = eps0 = 0.005 with lambda1 from 0.001 to 1000: the combinations'
+    potentials drift apart by up to 86 / k (measured with an FP64 log-domain
+    iteration on the CPU), beyond the 64 / k the factored sums accept, so those
+    tiles are recomputed exactly (C max / eps = 604: the oracle's K stays
+    representable in FP64)."""
+    coords, growth = _course(8, [130, 170])
+    res = batch.solve_time_course(coords, growth, epsilons=(0.005,), lambda1s=(0.001, 1.0, 1000.0),
+                                  epsilon0=0.005, graph_iters=0, **SCHED)
+    assert res.stats['shared_group'] == 3
+    print('exact tiles %d' % res.stats['exact_tiles'])
+    assert res.stats['exact_tiles'] > 0
+    _check(res, coords, growth, 0.005)
 
 
 def test_batch_graph_replay_is_deterministic(batch):
@@ -69,6 +99,12 @@ def test_batch_fallback_rows(batch):
                                   graph_iters=0, **SCHED)
     C = O.default_cost_matrix(coords[0], coords[1])
     assert C[0].min() / 0.01 * 1.4427 > 149 and C[0].min() / 0.01 < 700
+    assert res.stats['exact_tiles'] > 0
+    _check(res, coords, growth, 0.01)
+    # the same outlier with the shared-exponential kernel (3 combinations of eps 0.01)
+    res = batch.solve_time_course(coords, growth, epsilons=(0.01,), lambda1s=(0.5, 1.0, 5.0), epsilon0=0.01,
+                                  graph_iters=0, **SCHED)
+    assert res.stats['shared_group'] == 3 and res.stats['exact_tiles'] > 0
     _check(res, coords, growth, 0.01)
 
 

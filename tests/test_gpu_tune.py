@@ -64,13 +64,15 @@ def test_every_batch_variant_matches_oracle(tune):
         C = O.default_cost_matrix(coords[p], coords[p + 1])
         for e, (eps, lam1) in enumerate([(a, b) for a in grid['epsilons'] for b in grid['lambda1s']]):
             refs[p, e] = O.transport_stablev2(C, lam1, 50.0, eps, G=growth[p], epsilon0=1.0, **SCHED)
-    for tile in tune.variants(tune.BATCH_SWEEP):
-        tune.set_tile(tune.BATCH_SWEEP, -1, tile)
-        res = batch.solve_time_course(coords, growth, graph_iters=50, **grid, **SCHED)
-        for (p, e), Rref in refs.items():
-            R = res.plan(p, e)
-            err = np.linalg.norm(R - Rref) / np.linalg.norm(Rref)
-            assert err <= 1e-5, (tile, p, e, err)
+    for kernel, share in ((tune.BATCH_SWEEP, False), (tune.BATCH_SHARED, True)):
+        for tile in tune.variants(kernel):
+            tune.set_tile(kernel, -1, tile)
+            res = batch.solve_time_course(coords, growth, graph_iters=50, share_exp=share, **grid, **SCHED)
+            assert res.stats['shared_group'] == (2 if share else 0)
+            for (p, e), Rref in refs.items():
+                R = res.plan(p, e)
+                err = np.linalg.norm(R - Rref) / np.linalg.norm(Rref)
+                assert err <= 1e-5, (kernel, tile, p, e, err)
     tune.reset()
 
 
@@ -100,8 +102,13 @@ def test_autotune_time_course(tune):
     from wot_amd.ot import batch
     coords, growth = _course(6, [300, 420, 380])
     tc = batch.prepare_time_course(coords, growth)
-    r = tune.autotune_time_course(tc, (0.05, 0.1), (1.0,), iters=4, repeats=1)
+    r = tune.autotune_time_course(tc, (0.05, 0.1), (1.0,), iters=4, repeats=1)  # no equal epsilons
+    assert r.kernel == tune.BATCH_SWEEP
     assert set(r.timings) == set(tune.variants(tune.BATCH_SWEEP))
     assert tune.get_tile(tune.BATCH_SWEEP, r.buckets[0]) == r.best
     assert r.buckets[0] == tune.bucket(tune.BATCH_SWEEP, 0, tc.mean_row_length(), 2)
+    r = tune.autotune_time_course(tc, (0.05,), (1.0, 5.0), iters=4, repeats=1)  # one group of 2
+    assert r.kernel == tune.BATCH_SHARED
+    assert set(r.timings) == set(tune.variants(tune.BATCH_SHARED))
+    assert tune.get_tile(tune.BATCH_SHARED, r.buckets[0]) == r.best
     tune.reset()

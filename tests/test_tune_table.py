@@ -14,7 +14,7 @@ def clean_table():
 
 
 def test_variants_default_first_and_distinct():
-    for k in (tune.ROW_SWEEP, tune.BATCH_SWEEP):
+    for k in (tune.ROW_SWEEP, tune.BATCH_SWEEP, tune.BATCH_SHARED):
         v = tune.variants(k)
         assert len(v) == len(set(v)) >= 4
         assert all(r >= 1 and t % 64 == 0 for r, t in v)
@@ -30,6 +30,8 @@ def test_bucket_arithmetic():
     assert tune.bucket(tune.BATCH_SWEEP, 0, 10000, 99) == 15 * 32 + 13   # combinations clamp to 16
     L = _native.lib()
     assert L.wotk_num_buckets(tune.ROW_SWEEP) == 1024 and L.wotk_num_buckets(tune.BATCH_SWEEP) == 512
+    assert L.wotk_num_buckets(tune.BATCH_SHARED) == 512
+    assert tune.bucket(tune.BATCH_SHARED, 0, 10000, 9) == tune.bucket(tune.BATCH_SWEEP, 0, 10000, 9)
     assert L.wotk_shape_bucket(7, 1, 1, 1) == -1
 
 

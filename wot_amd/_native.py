@@ -22,6 +22,8 @@ COST_MODES = {'f64': COST_F64, 'bf16x3': COST_BF16X3, 'bf16x6': COST_BF16X6}
 
 KERNEL_ROW_SWEEP = 0
 KERNEL_BATCH_SWEEP = 1
+KERNEL_BATCH_SHARED = 2
+BATCH_SHARE_EXP = 1
 
 
 class Tile(ctypes.Structure):
@@ -75,7 +77,7 @@ SIGNATURES = {
     'wotk_plan': (ctypes.c_int, [_P, _I64, _I64, _I64, _P, _P, _D, _D, _P, _I64, _P, _P]),
     'wotk_batch_workspace_size': (_SZ, [_P, _I32, _I32, _I64]),
     'wotk_batch_stablev2': (ctypes.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _D, _I64, _D, _D, _I64, _I64, _P, _P,
-                                           _I32, _P, _SZ, ctypes.POINTER(Stats), _P]),
+                                           _I32, _I32, _P, _SZ, ctypes.POINTER(Stats), _P]),
     'wotk_fused_workspace_size': (_SZ, [_I64, _I64, _I64]),
     'wotk_transport_stablev2_coords': (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64, _D, _D, _I64,
                                                       _I64, _I32, _P, _P, _P, _P, _P, _SZ, ctypes.POINTER(Stats),

@@ -67,6 +67,14 @@ struct BatchArgs {
     const double *logq;        // per-pair log q (target side)
     const double *log_n_other; // per side-pair log(#columns)
     int *queue;                // work-queue counter
+    float *fac;                // shared kernel: this side's column factors ([E][pot_ld], like pot)
+    const float *fac_other;    // shared kernel: the other side's column factors
+    int k_next;                // shared kernel: factors are read at iteration it + k_next
+    unsigned long long *exact_tiles;  // count of tiles recomputed exactly
+    int *bad_list;             // tiles left to the repair kernel (indices into tiles)
+    int *n_bad;                // their count (reset every iteration)
+    int tile_rows;             // rows per tile
+    int group;                 // shared kernel: combinations per group (factors written); 0 otherwise
 };
 
 __device__ __forceinline__ float bexp2(float x) { return __builtin_amdgcn_exp2f(x); }
@@ -80,6 +88,100 @@ __device__ __forceinline__ float4 bload4(const float *p) {
     return v;
 }
 
+// Exact recomputation of every (row, combination) of a tile with a per-lane
+// online maximum (integer shifts, ldexp merges in FP64); writes
+// l2[r * E + e] = log2 sum_j 2^((h^e_j - C_ij) k_e).  Used when a tile's
+// fixed-shift sums under- or overflow.  Every thread of the block calls it.
+// One row of a tile (row < nrows), all E combinations; k of combination e is
+// prm[e].k.  Every thread of the block calls it (it synchronises).
+template <int E, int THREADS>
+__device__ __forceinline__ void row_exact(const float *A, long long lda, int row, int ncols, const double *h,
+                                          long long other_ld, const IterParam *prm, double *red_a, double *red_b,
+                                          double *l2) {
+    constexpr int NW = THREADS / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {
+        const float *rowp = A + (long long)row * lda;
+        double sig[E];
+        float s[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            sig[e] = -1.0e6;
+            s[e] = 0.f;
+        }
+        for (int c = tid * 4; c < ncols; c += THREADS * 4) {
+            const float4 a = bload4(rowp + c);
+            const double av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const double k = prm[e].k;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double y = fma(-k, av[q], h[(long long)e * other_ld + c + q] * k);
+                    if (y > sig[e]) {
+                        const double ns = ceil(y);
+                        s[e] = ldexpf(s[e], (int)(sig[e] - ns));
+                        sig[e] = ns;
+                    }
+                    s[e] += bexp2((float)(y - sig[e]));
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            double sg = sig[e], sd = (double)s[e];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double og = __shfl_xor(sg, off), os = __shfl_xor(sd, off);
+                const double ng = fmax(sg, og);
+                sd = ldexp(sd, (int)(sg - ng)) + ldexp(os, (int)(og - ng));
+                sg = ng;
+            }
+            if (lane == 0) {
+                red_a[e * NW + wave] = sg;
+                red_b[e * NW + wave] = sd;
+            }
+        }
+        __syncthreads();
+        if (tid < E) {
+            double sg = red_a[tid * NW], sd = red_b[tid * NW];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) {
+                const double og = red_a[tid * NW + w], os = red_b[tid * NW + w];
+                const double ng = fmax(sg, og);
+                sd = ldexp(sd, (int)(sg - ng)) + ldexp(os, (int)(og - ng));
+                sg = ng;
+            }
+            l2[tid] = sg + log2(sd);
+        }
+        __syncthreads();
+    }
+}
+
+// sum the per-lane FP32 partials of every (row, combination) over the block:
+// an FP32 shuffle tree inside each wave (positive terms: < 4e-7 relative), FP64
+// across waves; result in red_a[..][0..NW) summed by the caller
+template <int RE, int NW>
+__device__ __forceinline__ void lane_sums(const float *s, double (*red_a)[NW]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < RE; ++i) {
+        float v = s[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) red_a[i][wave] = (double)v;
+    }
+}
+
+// new potential of (row, combination) from l2 = log2 sum_j 2^((h_j - C_ij) k)
+__device__ __forceinline__ double new_potential(const BatchArgs &ba, const PairDev &pd, const IterParam &ip, int row,
+                                                int side_pair, double l2) {
+    const double lse = l2 * kLn2;  // ln sum_j exp((h_j - C_ij)/eps)
+    const double lw = (pd.logw_off >= 0) ? ba.logw[pd.logw_off + row] : ba.logq[pd.pair];
+    const double alpha = ba.side == 0 ? ip.alpha1 : ip.alpha2;
+    return alpha * ip.eps * (lw + ba.log_n_other[side_pair] - lse);
+}
+
 template <int E, int ROWS, int THREADS>
 __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
     constexpr int NW = THREADS / 64;
@@ -88,9 +190,8 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
     __shared__ int tile_sh;
     __shared__ int bad_sh;
     __shared__ double red_a[RE][NW];
-    __shared__ double red_b[RE][NW];
     __shared__ double l2_sh[RE];  // log2 sum_j 2^((h_j - C_ij) k) per (row, combination)
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const int it = *ba.iter;
     const IterParam *prm = ba.table + (size_t)it * E;
     double kk[E];
@@ -100,7 +201,7 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
     for (;;) {
         if (tid == 0) tile_sh = atomicAdd(ba.queue, 1);
         __syncthreads();
-        const int t = tile_sh;
+        const int t = __builtin_amdgcn_readfirstlane(tile_sh);
         if (tid == 0) bad_sh = 0;
         __syncthreads();
         if (t >= ba.ntiles) break;
@@ -148,15 +249,7 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
                 }
             }
         }
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r)
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                float v = s[r][e];  // positive terms: an FP32 tree over 64 lanes loses < 4e-7
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if (lane == 0) red_a[r * E + e][wave] = (double)v;
-            }
+        lane_sums<RE, NW>(&s[0][0], red_a);
         __syncthreads();
         if (tid < RE) {
             double S = 0.0;
@@ -164,96 +257,211 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
             for (int w = 0; w < NW; ++w) S += red_a[tid][w];
             const int r = tid / E, e = tid % E;
             l2_sh[tid] = log2(S) - fk[r][e];
-            if ((S == 0.0 || isinf(S)) && tl.row0 + r < pd.nrows) atomicOr(&bad_sh, 1);
+            if (!(S > 0.0 && S < INFINITY) && tl.row0 + r < pd.nrows) atomicOr(&bad_sh, 1);
         }
         __syncthreads();
-        if (bad_sh) {
-            // rare: the fixed shift under- or overflowed a row; recompute every
-            // (row, combination) of the tile with a per-lane online maximum
-            double sig[ROWS][E];
-#pragma unroll
-            for (int r = 0; r < ROWS; ++r)
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    sig[r][e] = -1.0e6;
-                    s[r][e] = 0.f;
-                }
-            for (int c = tid * 4; c < pd.ncols; c += THREADS * 4) {
-#pragma unroll
-                for (int r = 0; r < ROWS; ++r) {
-                    const float4 a = bload4(rowp[r] + c);
-                    const double av[4] = {a.x, a.y, a.z, a.w};
-#pragma unroll
-                    for (int e = 0; e < E; ++e)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            const double y = fma(-kk[e], av[q], h[(long long)e * pd.other_ld + c + q] * kk[e]);
-                            if (y > sig[r][e]) {
-                                const double ns = ceil(y);
-                                s[r][e] = ldexpf(s[r][e], (int)(sig[r][e] - ns));
-                                sig[r][e] = ns;
-                            }
-                            s[r][e] += bexp2((float)(y - sig[r][e]));
-                        }
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < ROWS; ++r)
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    double sg = sig[r][e], sd = (double)s[r][e];
-#pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) {
-                        const double og = __shfl_xor(sg, off), os = __shfl_xor(sd, off);
-                        const double ng = fmax(sg, og);
-                        sd = ldexp(sd, (int)(sg - ng)) + ldexp(os, (int)(og - ng));
-                        sg = ng;
-                    }
-                    if (lane == 0) {
-                        red_a[r * E + e][wave] = sg;
-                        red_b[r * E + e][wave] = sd;
-                    }
-                }
-            __syncthreads();
-            if (tid < RE) {
-                double sg = red_a[tid][0], sd = red_b[tid][0];
-#pragma unroll
-                for (int w = 1; w < NW; ++w) {
-                    const double og = red_a[tid][w], os = red_b[tid][w];
-                    const double ng = fmax(sg, og);
-                    sd = ldexp(sd, (int)(sg - ng)) + ldexp(os, (int)(og - ng));
-                    sg = ng;
-                }
-                l2_sh[tid] = sg + log2(sd);
-            }
-            __syncthreads();
-        }
-        if (tid < RE) {
+        // rare: the fixed shift under- or overflowed a row; the repair kernel
+        // recomputes the tile exactly.  The flag is read as a scalar and no
+        // barrier sits under a condition, so every lane keeps the same path.
+        const int bad = __builtin_amdgcn_readfirstlane(bad_sh);
+        if (bad && tid == 0) ba.bad_list[atomicAdd(ba.n_bad, 1)] = t;
+        if (!bad && tid < RE) {
             const int r = tid / E, e = tid % E;
             const int row = tl.row0 + r;
-            if (row < pd.nrows) {
-                const IterParam ip = prm[e];
-                const double lse = l2_sh[tid] * kLn2;  // ln sum_j exp((h_j - C_ij)/eps)
-                const double lw = (pd.logw_off >= 0) ? ba.logw[pd.logw_off + row] : ba.logq[pd.pair];
-                const double alpha = ba.side == 0 ? ip.alpha1 : ip.alpha2;
-                pot[(long long)e * pd.pot_ld + row] = alpha * ip.eps * (lw + ba.log_n_other[tl.side_pair] - lse);
-            }
+            if (row < pd.nrows)
+                pot[(long long)e * pd.pot_ld + row] = new_potential(ba, pd, prm[e], row, tl.side_pair, l2_sh[tid]);
         }
         __syncthreads();
+    }
+}
+
+// Shared-exponential sweep for NG groups of NL combinations of equal epsilon
+// (combination e = g * NL + l).  Within a group the reference combination
+// l0 = NL / 2 gives the fixed-shift exponent
+//   y_ij = k (f^ref_i,old + h^ref_j - C_ij),   one exp2 per (entry, group),
+// and every combination of the group sums 2^y_ij * B^e_j with the column
+// factor B^e_j = 2^(k (h^e_j - h^ref_j)) (FP32, written by the other side's
+// epilogue), so that
+//   log2 sum_j 2^((h^e_j - C_ij) k) = log2 sum_j 2^y_ij B^e_j - k f^ref_i,old.
+// The terms are combination e's previous plan entries scaled by
+// 2^(-k (f^e_i - f^ref_i)); a tile whose row offsets exceed 2^64 either way, or
+// whose sums under- or overflow (B = inf or 0 included), is recomputed exactly.
+template <int NG, int NL, int ROWS, int THREADS>
+__global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
+    constexpr int E = NG * NL, L0 = NL / 2, NW = THREADS / 64, RE = ROWS * E;
+    static_assert(RE <= THREADS, "one thread per (row, combination) in the epilogue");
+    __shared__ int tile_sh;
+    __shared__ int bad_sh;
+    __shared__ double red_a[RE][NW];
+    __shared__ double l2_sh[RE];
+    __shared__ double pot_sh[RE];
+    const int tid = threadIdx.x;
+    const int it = *ba.iter;
+    const IterParam *prm = ba.table + (size_t)it * E;
+    const IterParam *nxt = ba.table + (size_t)(it + ba.k_next) * E;  // k the factors are read with
+    double kg[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) kg[g] = prm[g * NL + L0].k;
+
+    for (;;) {
+        if (tid == 0) tile_sh = atomicAdd(ba.queue, 1);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(tile_sh);
+        if (tid == 0) bad_sh = 0;
+        __syncthreads();
+        if (t >= ba.ntiles) break;
+        const Tile tl = ba.tiles[t];
+        const PairDev pd = ba.pairs[tl.side_pair];
+        const float *A = ba.cost + pd.a_off;
+        const double *h = ba.other + pd.other_off;
+        const float *B = ba.fac_other + pd.other_off;
+        double *pot = ba.pot + pd.pot_off;
+        const float *rowp[ROWS];
+        double fk[ROWS][NG];
+        float s[ROWS][E];
+        bool far = false;
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const int row = min(tl.row0 + r, pd.nrows - 1);
+            rowp[r] = A + (long long)row * pd.lda;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                fk[r][g] = pot[(long long)(g * NL + L0) * pd.pot_ld + row] * kg[g];
+#pragma unroll
+                for (int l = 0; l < NL; ++l) {
+                    const double d = pot[(long long)(g * NL + l) * pd.pot_ld + row] * kg[g] - fk[r][g];
+                    far |= !(fabs(d) <= 64.0);
+                    s[r][g * NL + l] = 0.f;
+                }
+            }
+        }
+        for (int c = tid * 4; c < pd.ncols; c += THREADS * 4) {
+            double cv[ROWS][4];
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) {
+                const float4 a = bload4(rowp[r] + c);
+                cv[r][0] = a.x;
+                cv[r][1] = a.y;
+                cv[r][2] = a.z;
+                cv[r][3] = a.w;
+            }
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const double *he = h + (long long)(g * NL + L0) * pd.other_ld + c;
+                const double2 h01 = *reinterpret_cast<const double2 *>(he);
+                const double2 h23 = *reinterpret_cast<const double2 *>(he + 2);
+                const double hk[4] = {h01.x * kg[g], h01.y * kg[g], h23.x * kg[g], h23.y * kg[g]};
+                float4 bf[NL];
+#pragma unroll
+                for (int l = 0; l < NL; ++l)
+                    if (l != L0) bf[l] = *reinterpret_cast<const float4 *>(B + (long long)(g * NL + l) * pd.other_ld + c);
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r) {
+                    float tq[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) tq[q] = bexp2((float)fma(-kg[g], cv[r][q], hk[q] + fk[r][g]));
+                    s[r][g * NL + L0] += (tq[0] + tq[1]) + (tq[2] + tq[3]);
+#pragma unroll
+                    for (int l = 0; l < NL; ++l)
+                        if (l != L0)
+                            s[r][g * NL + l] += fmaf(tq[0], bf[l].x, fmaf(tq[1], bf[l].y, fmaf(tq[2], bf[l].z, tq[3] * bf[l].w)));
+                }
+            }
+        }
+        lane_sums<RE, NW>(&s[0][0], red_a);
+        if (far) bad_sh = 1;  // benign race: every writer stores 1
+        __syncthreads();
+        if (tid < RE) {
+            double S = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) S += red_a[tid][w];
+            const int r = tid / E, g = (tid % E) / NL;
+            l2_sh[tid] = log2(S) - fk[r][g];
+            if (!(S > 0.0 && S < INFINITY) && tl.row0 + r < pd.nrows) atomicOr(&bad_sh, 1);
+        }
+        __syncthreads();
+        // the flag is read as a scalar and no barrier sits under a condition
+        const int bad = __builtin_amdgcn_readfirstlane(bad_sh);
+        if (bad && tid == 0) ba.bad_list[atomicAdd(ba.n_bad, 1)] = t;
+        if (!bad && tid < RE) {
+            const int r = tid / E, e = tid % E;
+            const int row = tl.row0 + r;
+            const double p = new_potential(ba, pd, prm[e], min(row, pd.nrows - 1), tl.side_pair, l2_sh[tid]);
+            pot_sh[tid] = p;
+            if (row < pd.nrows) pot[(long long)e * pd.pot_ld + row] = p;
+        }
+        __syncthreads();
+        if (!bad && tid < RE) {
+            // this side's factors, read by the other side's next sweep with k of that sweep
+            const int r = tid / E, e = tid % E, g = e / NL;
+            const int row = tl.row0 + r;
+            if (row < pd.nrows)
+                ba.fac[pd.pot_off + (long long)e * pd.pot_ld + row] =
+                    (float)exp2(nxt[g * NL + L0].k * (pot_sh[tid] - pot_sh[r * E + g * NL + L0]));
+        }
+        __syncthreads();
+    }
+}
+
+// Exact recomputation (per-lane online maximum) of the tiles a sweep queued,
+// row by row, with the sweep's epilogue: potentials and, for the
+// shared-exponential kernel, the column factors.  Runs after every sweep; with
+// no queued tile it exits at once.
+template <int E, int THREADS>
+__global__ __launch_bounds__(THREADS) void batch_repair_kernel(BatchArgs ba) {
+    constexpr int NW = THREADS / 64;
+    __shared__ double red_a[E * NW];
+    __shared__ double red_b[E * NW];
+    __shared__ double l2_sh[E];
+    __shared__ double pot_sh[E];
+    const int tid = threadIdx.x;
+    const int nb = __builtin_amdgcn_readfirstlane(*ba.n_bad);
+    if ((int)blockIdx.x >= nb) return;
+    const int it = *ba.iter;
+    const IterParam *prm = ba.table + (size_t)it * E;
+    const IterParam *nxt = ba.table + (size_t)(it + ba.k_next) * E;
+    if (blockIdx.x == 0 && tid == 0) atomicAdd(ba.exact_tiles, (unsigned long long)nb);
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const Tile tl = ba.tiles[ba.bad_list[b]];
+        const PairDev pd = ba.pairs[tl.side_pair];
+        const float *A = ba.cost + pd.a_off;
+        const double *h = ba.other + pd.other_off;
+        double *pot = ba.pot + pd.pot_off;
+        const int rend = min(tl.row0 + ba.tile_rows, pd.nrows);
+        for (int row = tl.row0; row < rend; ++row) {
+            row_exact<E, THREADS>(A, pd.lda, row, pd.ncols, h, pd.other_ld, prm, red_a, red_b, l2_sh);
+            if (tid < E) {
+                const double p = new_potential(ba, pd, prm[tid], row, tl.side_pair, l2_sh[tid]);
+                pot_sh[tid] = p;
+                pot[(long long)tid * pd.pot_ld + row] = p;
+            }
+            __syncthreads();
+            if (ba.group && tid < E) {
+                const int ref = (tid / ba.group) * ba.group + ba.group / 2;
+                ba.fac[pd.pot_off + (long long)tid * pd.pot_ld + row] =
+                    (float)exp2(nxt[ref].k * (pot_sh[tid] - pot_sh[ref]));
+            }
+            __syncthreads();
+        }
     }
 }
 
 // advance the iteration counter and reset the two work queues (one thread)
 __global__ void batch_advance_kernel(int *iter, int *queues) {
     iter[0] += 1;
-    queues[0] = 0;
+    queues[0] = 0;  // work queues of the two sides
     queues[1] = 0;
+    queues[2] = 0;  // repair counts of the two sides
+    queues[3] = 0;
 }
 
-__global__ void batch_reset_kernel(int *iter, int *queues, int start) {
+__global__ void batch_reset_kernel(int *iter, int *queues, unsigned long long *exact_tiles, int start) {
     iter[0] = start;
     queues[0] = 0;
     queues[1] = 0;
+    queues[2] = 0;
+    queues[3] = 0;
+    exact_tiles[0] = 0;
 }
 
 __global__ void batch_fill_kernel(double *x, long long n, double v) {
@@ -293,14 +501,29 @@ __global__ void batch_pad_kernel(double *__restrict__ pot, const PairDev *__rest
     }
 }
 
+// factors of the shared-exponential kernel: 1 on the rows (potentials start
+// equal across combinations), 0 on the padding rows [n, n_pad)
+__global__ void batch_fac_init_kernel(float *__restrict__ fac, const PairDev *__restrict__ pairs, int n_pairs, int E) {
+    const int p = blockIdx.x;
+    if (p >= n_pairs) return;
+    const PairDev pd = pairs[p];
+    for (int t = threadIdx.x; t < E * pd.pot_ld; t += blockDim.x) {
+        const int i = t % pd.pot_ld;
+        fac[pd.pot_off + t] = i < pd.nrows ? 1.f : 0.f;
+    }
+}
+
 struct BatchWork {
     double *f, *g, *logw, *logq, *lognf, *logng;
+    float *ff, *fg;  // column factors of the shared-exponential kernel
     IterParam *table;
     PairDev *pairs_f, *pairs_g;
     Tile *tiles_f, *tiles_g;
     long long *g_off, *lw_off;
     int *n_src;
     int *iter, *queues;
+    unsigned long long *exact_tiles;
+    int *bad_f, *bad_g;  // repair lists
 };
 
 }  // namespace wotk
@@ -319,6 +542,8 @@ static size_t batch_carve(Arena &ar, const wotk_batch_pair *pairs, int n_pairs, 
     }
     w->f = ar.take<double>((size_t)nf * E);
     w->g = ar.take<double>((size_t)ng * E);
+    w->ff = ar.take<float>((size_t)nf * E);
+    w->fg = ar.take<float>((size_t)ng * E);
     w->logw = ar.take<double>(nlw);
     w->logq = ar.take<double>(n_pairs);
     w->lognf = ar.take<double>(n_pairs);
@@ -328,11 +553,14 @@ static size_t batch_carve(Arena &ar, const wotk_batch_pair *pairs, int n_pairs, 
     w->pairs_g = ar.take<PairDev>(n_pairs);
     w->tiles_f = ar.take<Tile>(tf);
     w->tiles_g = ar.take<Tile>(tg);
+    w->bad_f = ar.take<int>(tf);
+    w->bad_g = ar.take<int>(tg);
     w->g_off = ar.take<long long>(n_pairs);
     w->lw_off = ar.take<long long>(n_pairs);
     w->n_src = ar.take<int>(n_pairs);
     w->iter = ar.take<int>(4);
     w->queues = ar.take<int>(4);
+    w->exact_tiles = ar.take<unsigned long long>(1);
     *nf_out = nf;
     *ng_out = ng;
     *ntf = tf;
@@ -349,7 +577,8 @@ extern "C" size_t wotk_batch_workspace_size(const wotk_batch_pair *pairs, int32_
 }
 
 // launchers per (combinations, tile variant); variants in the order of kBatchSweepTiles
-typedef int (*SweepFn)(const BatchArgs &, const BatchArgs &, int, int, int *, int *, hipStream_t);
+typedef int (*SweepFn)(const BatchArgs &, const BatchArgs &, int, int, int, int *, int *, hipStream_t);
+constexpr int kRepairThreads = 256;
 typedef int (*OccFn)();  // resident workgroups per CU
 
 struct BatchVariant {
@@ -358,11 +587,13 @@ struct BatchVariant {
 };
 
 template <int E, int V>
-static int launch_sweeps(const BatchArgs &af, const BatchArgs &ag, int grid_f, int grid_g, int *iter, int *queues,
-                         hipStream_t s) {
+static int launch_sweeps(const BatchArgs &af, const BatchArgs &ag, int grid_f, int grid_g, int grid_r, int *iter,
+                         int *queues, hipStream_t s) {
     constexpr int R = kBatchSweepTiles[V].rows, T = kBatchSweepTiles[V].threads;
     batch_sweep_kernel<E, R, T><<<grid_f, T, 0, s>>>(af);
+    batch_repair_kernel<E, kRepairThreads><<<grid_r, kRepairThreads, 0, s>>>(af);
     batch_sweep_kernel<E, R, T><<<grid_g, T, 0, s>>>(ag);
+    batch_repair_kernel<E, kRepairThreads><<<grid_r, kRepairThreads, 0, s>>>(ag);
     batch_advance_kernel<<<1, 1, 0, s>>>(iter, queues);
     WOTK_LAUNCH_CHECK();
     return WOTK_OK;
@@ -402,6 +633,54 @@ static const BatchVariant *pick(int E) {
     }
 }
 
+// shared-exponential launchers per (groups, group size, tile variant of kBatchSharedTiles)
+template <int NG, int NL, int V>
+static int launch_shared(const BatchArgs &af, const BatchArgs &ag, int grid_f, int grid_g, int grid_r, int *iter,
+                         int *queues, hipStream_t s) {
+    constexpr int R = kBatchSharedTiles[V].rows, T = kBatchSharedTiles[V].threads;
+    batch_shared_kernel<NG, NL, R, T><<<grid_f, T, 0, s>>>(af);
+    batch_repair_kernel<NG * NL, kRepairThreads><<<grid_r, kRepairThreads, 0, s>>>(af);
+    batch_shared_kernel<NG, NL, R, T><<<grid_g, T, 0, s>>>(ag);
+    batch_repair_kernel<NG * NL, kRepairThreads><<<grid_r, kRepairThreads, 0, s>>>(ag);
+    batch_advance_kernel<<<1, 1, 0, s>>>(iter, queues);
+    WOTK_LAUNCH_CHECK();
+    return WOTK_OK;
+}
+
+template <int NG, int NL, int V>
+static int occupancy_shared() {
+    constexpr int R = kBatchSharedTiles[V].rows, T = kBatchSharedTiles[V].threads;
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_shared_kernel<NG, NL, R, T>, T, 0) != hipSuccess)
+        return 1;
+    return per < 1 ? 1 : per;
+}
+
+template <int NG, int NL, int... V>
+static constexpr std::array<BatchVariant, sizeof...(V)> shared_variants_of(std::integer_sequence<int, V...>) {
+    return {BatchVariant{launch_shared<NG, NL, V>, occupancy_shared<NG, NL, V>}...};
+}
+
+template <int NG, int NL>
+static const BatchVariant *shared_variants() {
+    static constexpr auto t = shared_variants_of<NG, NL>(std::make_integer_sequence<int, kNumBatchSharedTiles>());
+    return t.data();
+}
+
+// compiled (groups, group size) shapes of the shared-exponential kernel
+static const BatchVariant *pick_shared(int ng, int nl) {
+    switch (ng * 16 + nl) {
+        case 1 * 16 + 2: return shared_variants<1, 2>();
+        case 1 * 16 + 3: return shared_variants<1, 3>();
+        case 2 * 16 + 2: return shared_variants<2, 2>();
+        case 2 * 16 + 3: return shared_variants<2, 3>();
+        case 3 * 16 + 2: return shared_variants<3, 2>();
+        case 3 * 16 + 3: return shared_variants<3, 3>();
+        case 4 * 16 + 4: return shared_variants<4, 4>();
+        default: return nullptr;
+    }
+}
+
 static int persistent_grid(const BatchVariant &v) {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 1024;
@@ -413,7 +692,8 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
                                    int32_t n_combos, const double *epsilon, const double *lambda1, double lambda2,
                                    int64_t scaling_iter, double tau, double epsilon0, int64_t extra_iter,
                                    int64_t inner_iter_max, double *f_out, double *g_out, int32_t graph_iters,
-                                   void *workspace, size_t workspace_bytes, wotk_stats *stats, void *stream) {
+                                   int32_t flags, void *workspace, size_t workspace_bytes, wotk_stats *stats,
+                                   void *stream) {
     hipStream_t s = (hipStream_t)stream;
     const int E = n_combos;
     WOTK_REQUIRE(n_pairs > 0 && pairs && cost && G, "empty batch");
@@ -441,9 +721,27 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
         entries += 2.0 * (double)pairs[p].n_i * (double)pairs[p].n_j;
         rows_total += (double)pairs[p].n_i + (double)pairs[p].n_j;
     }
-    const int vi = tile_for(WOTK_KERNEL_BATCH_SWEEP, 0, (int64_t)(entries / rows_total), E);
-    const BatchVariant &variant = pick(E)[vi];
-    const int tile_rows = kBatchSweepTiles[vi].rows;
+    // shared exponentials: the largest group size of consecutive equal epsilons with a compiled shape
+    int group = 0;
+    if (flags & WOTK_BATCH_SHARE_EXP)
+        for (int nl = E; nl >= 2 && !group; --nl) {
+            if (E % nl || !pick_shared(E / nl, nl)) continue;
+            bool equal = true;
+            for (int e = 0; e < E; ++e) equal = equal && epsilon[e] == epsilon[e - e % nl];
+            if (equal) group = nl;
+        }
+    const int64_t mean_len = (int64_t)(entries / rows_total);
+    int vi, tile_rows;
+    const BatchVariant *variant;
+    if (group) {
+        vi = tile_for(WOTK_KERNEL_BATCH_SHARED, 0, mean_len, E);
+        variant = &pick_shared(E / group, group)[vi];
+        tile_rows = kBatchSharedTiles[vi].rows;
+    } else {
+        vi = tile_for(WOTK_KERNEL_BATCH_SWEEP, 0, mean_len, E);
+        variant = &pick(E)[vi];
+        tile_rows = kBatchSweepTiles[vi].rows;
+    }
 
     // host-side tables: pairs, tiles (longest rows first), iteration parameters
     std::vector<PairDev> pf(n_pairs), pg(n_pairs);
@@ -500,18 +798,26 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     WOTK_LAUNCH_CHECK();
     batch_pad_kernel<<<n_pairs, 64, 0, s>>>(w.f, w.pairs_f, n_pairs, E);
     batch_pad_kernel<<<n_pairs, 64, 0, s>>>(w.g, w.pairs_g, n_pairs, E);
+    batch_fac_init_kernel<<<n_pairs, 256, 0, s>>>(w.ff, w.pairs_f, n_pairs, E);
+    batch_fac_init_kernel<<<n_pairs, 256, 0, s>>>(w.fg, w.pairs_g, n_pairs, E);
     WOTK_LAUNCH_CHECK();
     batch_growth_kernel<<<n_pairs, 256, 0, s>>>(G, w.g_off, w.n_src, w.logw, w.lw_off, w.logq);
-    batch_reset_kernel<<<1, 1, 0, s>>>(w.iter, w.queues, 0);
+    batch_reset_kernel<<<1, 1, 0, s>>>(w.iter, w.queues, w.exact_tiles, 0);
     WOTK_LAUNCH_CHECK();
     WOTK_HIP(hipStreamSynchronize(s));  // host tables above are released at return
 
     const int64_t n_tf = (int64_t)tf.size(), n_tg = (int64_t)tg.size();  // <= ntf, ntg (one-row tiles)
-    BatchArgs af{cost, w.pairs_f, w.tiles_f, (int)n_tf, w.table, w.iter, E, 0, w.f, w.g, w.logw, w.logq, w.lognf, w.queues};
-    BatchArgs ag{cost, w.pairs_g, w.tiles_g, (int)n_tg, w.table, w.iter, E, 1, w.g, w.f, w.logw, w.logq, w.logng, w.queues + 1};
-    const int grid = persistent_grid(variant);
+    BatchArgs af{cost, w.pairs_f, w.tiles_f, (int)n_tf, w.table, w.iter, E, 0, w.f, w.g, w.logw, w.logq, w.lognf,
+                 w.queues, w.ff, w.fg, 0, w.exact_tiles, w.bad_f, w.queues + 2, tile_rows, group};
+    BatchArgs ag{cost, w.pairs_g, w.tiles_g, (int)n_tg, w.table, w.iter, E, 1, w.g, w.f, w.logw, w.logq, w.logng,
+                 w.queues + 1, w.fg, w.ff, 1, w.exact_tiles, w.bad_g, w.queues + 3, tile_rows, group};
+    const int grid = persistent_grid(*variant);
     const int grid_f = (int)std::min<int64_t>(grid, n_tf), grid_g = (int)std::min<int64_t>(grid, n_tg);
-    const SweepFn sweep = variant.sweep;
+    const SweepFn sweep = variant->sweep;
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 256;
+    const int grid_r = ncu;  // repair blocks: at most this many queued tiles in flight
 
     hipEvent_t ev[2];
     WOTK_HIP(hipEventCreate(&ev[0]));
@@ -525,7 +831,7 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
         WOTK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         WOTK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         for (int t = 0; t < graph_iters; ++t) {
-            int rc = sweep(af, ag, grid_f, grid_g, w.iter, w.queues, cs);
+            int rc = sweep(af, ag, grid_f, grid_g, grid_r, w.iter, w.queues, cs);
             if (rc) {
                 hipStreamEndCapture(cs, &graph);
                 return rc;
@@ -542,7 +848,7 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
         }
     }
     for (; done < n_iter; ++done) {
-        int rc = sweep(af, ag, grid_f, grid_g, w.iter, w.queues, s);
+        int rc = sweep(af, ag, grid_f, grid_g, grid_r, w.iter, w.queues, s);
         if (rc) return rc;
     }
     WOTK_HIP(hipEventRecord(ev[1], s));
@@ -564,6 +870,8 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
             ob += (long long)E * pairs[p].n_j;
         }
     }
+    unsigned long long exact_tiles = 0;
+    WOTK_HIP(hipMemcpyAsync(&exact_tiles, w.exact_tiles, sizeof(exact_tiles), hipMemcpyDeviceToHost, s));
     WOTK_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
     WOTK_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
@@ -578,6 +886,8 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
         stats->sweeps = 2 * n_iter;
         stats->sweep_ms = ms;
         stats->epsilon_final = table[(size_t)n_iter * E].eps;
+        stats->stabilisations = (int64_t)exact_tiles;
+        stats->gap_checks = group;
     }
     return WOTK_OK;
 }

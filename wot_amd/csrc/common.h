@@ -50,6 +50,8 @@ constexpr TileShape kRowSweepTiles[] = {{4, 256}, {1, 256}, {2, 256}, {8, 256},
 constexpr int kNumRowSweepTiles = sizeof(kRowSweepTiles) / sizeof(kRowSweepTiles[0]);
 constexpr TileShape kBatchSweepTiles[] = {{2, 256}, {1, 256}, {4, 256}, {1, 512}, {2, 512}};
 constexpr int kNumBatchSweepTiles = sizeof(kBatchSweepTiles) / sizeof(kBatchSweepTiles[0]);
+constexpr TileShape kBatchSharedTiles[] = {{2, 256}, {1, 256}, {4, 256}, {2, 512}};
+constexpr int kNumBatchSharedTiles = sizeof(kBatchSharedTiles) / sizeof(kBatchSharedTiles[0]);
 
 // variant index (into the lists above) a launch of this shape uses
 int tile_for(int kernel, int64_t n_rows, int64_t n_cols, int n_combos);

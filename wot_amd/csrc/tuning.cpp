@@ -12,11 +12,13 @@ constexpr int kBatchBuckets = 16 * 32; // combinations 1..16 x floor(log2 mean r
 // variant index per bucket, -1 = default (variant 0)
 std::atomic<int> g_row[kRowBuckets];
 std::atomic<int> g_batch[kBatchBuckets];
+std::atomic<int> g_shared[kBatchBuckets];
 
 struct TableInit {
     TableInit() {
         for (auto &v : g_row) v.store(-1);
         for (auto &v : g_batch) v.store(-1);
+        for (auto &v : g_shared) v.store(-1);
     }
 } g_init;
 
@@ -42,6 +44,12 @@ std::atomic<int> *table(int kernel, int *n_buckets, const wotk::TileShape **tile
         *n_tiles = wotk::kNumBatchSweepTiles;
         return g_batch;
     }
+    if (kernel == WOTK_KERNEL_BATCH_SHARED) {
+        *n_buckets = kBatchBuckets;
+        *tiles = wotk::kBatchSharedTiles;
+        *n_tiles = wotk::kNumBatchSharedTiles;
+        return g_shared;
+    }
     return nullptr;
 }
 
@@ -49,7 +57,7 @@ std::atomic<int> *table(int kernel, int *n_buckets, const wotk::TileShape **tile
 
 extern "C" int32_t wotk_shape_bucket(int32_t kernel, int64_t n_rows, int64_t n_cols, int32_t n_combos) {
     if (kernel == WOTK_KERNEL_ROW_SWEEP) return lg(n_rows) * 32 + lg(n_cols);
-    if (kernel == WOTK_KERNEL_BATCH_SWEEP) {
+    if (kernel == WOTK_KERNEL_BATCH_SWEEP || kernel == WOTK_KERNEL_BATCH_SHARED) {
         const int e = n_combos < 1 ? 1 : (n_combos > 16 ? 16 : n_combos);
         return (e - 1) * 32 + lg(n_cols);
     }

@@ -130,20 +130,22 @@ def prepare_time_course(coords, growth, pairs=None, cost_mode='f64', device='cud
 
 def solve_time_course(coords, growth, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_iter=3000,
                       tau=1e4, epsilon0=1.0, extra_iter=1000, inner_iter_max=50, graph_iters=50,
-                      cost_mode='f64', pairs=None, group_size=None, device='cuda'):
+                      cost_mode='f64', pairs=None, group_size=None, share_exp=True, device='cuda'):
     """Solve every consecutive day pair (or the given index pairs) for every
     (epsilon, lambda1) of the grid epsilons x lambda1s.  ``coords[t]`` are the
     PCA coordinates of day t, ``growth[t]`` its cells' growth (G = growth, dt=1).
     ``group_size`` splits the grid into batched runs of that many combinations
-    (each cost entry is then read once per group instead of once overall)."""
+    (each cost entry is then read once per group instead of once overall).
+    ``share_exp`` lets combinations of equal epsilon share one exponential per
+    cost entry (libwotk ``WOTK_BATCH_SHARE_EXP``)."""
     tc = prepare_time_course(coords, growth, pairs=pairs, cost_mode=cost_mode, device=device)
     return solve_prepared(tc, epsilons, lambda1s, lambda2=lambda2, scaling_iter=scaling_iter, tau=tau,
                           epsilon0=epsilon0, extra_iter=extra_iter, inner_iter_max=inner_iter_max,
-                          graph_iters=graph_iters, group_size=group_size)
+                          graph_iters=graph_iters, group_size=group_size, share_exp=share_exp)
 
 
 def solve_prepared(tc, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_iter=3000, tau=1e4, epsilon0=1.0,
-                   extra_iter=1000, inner_iter_max=50, graph_iters=50, group_size=None):
+                   extra_iter=1000, inner_iter_max=50, graph_iters=50, group_size=None, share_exp=True):
     """``solve_time_course`` on costs already built by ``prepare_time_course``."""
     torch = _dev.torch_mod()
     from .. import tune
@@ -171,7 +173,8 @@ def solve_prepared(tc, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_
         _native.check(L.wotk_batch_stablev2(_dev._ptr(buf), arr, len(bps), _dev._ptr(G), Eg, eps, lam,
                                             float(lambda2), int(scaling_iter), float(tau), float(epsilon0),
                                             int(extra_iter), int(inner_iter_max), _dev._ptr(f_out), _dev._ptr(g_out),
-                                            int(graph_iters), _dev._ptr(ws), ws.numel(), ctypes.byref(st),
+                                            int(graph_iters), _native.BATCH_SHARE_EXP if share_exp else 0,
+                                            _dev._ptr(ws), ws.numel(), ctypes.byref(st),
                                             _dev._stream(torch)))
         a = b = 0
         for k, (n_i, n_j) in enumerate(shapes):
@@ -180,11 +183,15 @@ def solve_prepared(tc, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_
             a += Eg * n_i
             b += Eg * n_j
         d = st.as_dict()
+        d['exact_tiles'] = d['stabilisations']  # tiles recomputed with the online maximum
+        d['shared_group'] = d['gap_checks']     # combinations per shared exponential (0: none)
         if st_total is None:
             st_total = d
         else:
             st_total['sweep_ms'] += d['sweep_ms']
             st_total['sweeps'] += d['sweeps']
+            st_total['stabilisations'] += d['stabilisations']
+            st_total['exact_tiles'] += d['exact_tiles']
 
     def reg(e, n):
         return (epsilon0 - e) * np.exp(-n) + e

@@ -26,7 +26,8 @@ from . import _native
 
 ROW_SWEEP = _native.KERNEL_ROW_SWEEP
 BATCH_SWEEP = _native.KERNEL_BATCH_SWEEP
-KERNEL_NAMES = {ROW_SWEEP: 'row_sweep', BATCH_SWEEP: 'batch_sweep'}
+BATCH_SHARED = _native.KERNEL_BATCH_SHARED
+KERNEL_NAMES = {ROW_SWEEP: 'row_sweep', BATCH_SWEEP: 'batch_sweep', BATCH_SHARED: 'batch_shared'}
 
 _lock = threading.Lock()
 _applied = False
@@ -192,17 +193,21 @@ def autotune_row_sweep(cost, G=None, epsilon=0.05, lambda1=1.0, lambda2=50.0, it
     return _pick(ROW_SWEEP, buckets, measure, repeats, persist)
 
 
-def autotune_time_course(tc, epsilons, lambda1s, lambda2=50.0, iters=20, repeats=2, persist=True):
+def autotune_time_course(tc, epsilons, lambda1s, lambda2=50.0, iters=20, repeats=2, share_exp=True, persist=True):
     """Tune the batched sweep on a prepared time course (``ot.batch.TimeCourseCosts``)
     for the grid epsilons x lambda1s: each variant runs ``iters`` iterations
-    of the whole batch; timed per half-iteration launch."""
+    of the whole batch; timed per half-iteration launch.  Tunes the kernel the
+    grid runs on (the shared-exponential one when combinations of equal
+    epsilon can share, else the per-combination one)."""
     from .ot import batch
     E = len(epsilons) * len(lambda1s)
-    buckets = [bucket(BATCH_SWEEP, 0, tc.mean_row_length(), E)]
+    kw = dict(lambda2=lambda2, extra_iter=0, inner_iter_max=50, graph_iters=0, share_exp=share_exp)
+    probe = batch.solve_prepared(tc, epsilons, lambda1s, scaling_iter=1, **kw)
+    kernel = BATCH_SHARED if probe.stats['shared_group'] else BATCH_SWEEP
+    buckets = [bucket(kernel, 0, tc.mean_row_length(), E)]
 
     def measure():
-        r = batch.solve_prepared(tc, epsilons, lambda1s, lambda2=lambda2, scaling_iter=int(iters), extra_iter=0,
-                                 inner_iter_max=50, graph_iters=0)
+        r = batch.solve_prepared(tc, epsilons, lambda1s, scaling_iter=int(iters), **kw)
         return 1e3 * r.stats['sweep_ms'] / max(1, r.stats['sweeps'])
 
-    return _pick(BATCH_SWEEP, buckets, measure, repeats, persist)
+    return _pick(kernel, buckets, measure, repeats, persist)
