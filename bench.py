@@ -246,7 +246,7 @@ def bench_timecourse(args):
                    'tiles': tiles},
         'sinkhorn_iters_per_s': n_solves * n_iter / elapsed,
         'sweep_loop_s': sweep_s,
-        'roofline': {'bound': 'valu/exp', 'kernel': 'batch_sweep_kernel<9>', 'avg_launch_us': avg_sweep * 1e6,
+        'roofline': {'bound': 'valu/exp', 'kernel': _tc_kernel_name(res.stats, tiles), 'avg_launch_us': avg_sweep * 1e6,
                      'hbm_achieved': alg_bytes / avg_sweep / 1e9, 'hbm_peak': HBM_PEAK_GBS,
                      'exp_per_s': n_combo * sum_ij / avg_sweep, 'unit': 'GB/s',
                      'achieved': alg_bytes / avg_sweep / 1e9, 'peak': HBM_PEAK_GBS,
@@ -270,6 +270,16 @@ def bench_timecourse(args):
     if not args.no_baselines:
         out.update(_timecourse_baselines(torch, coords, growth, pair_ij, n_combo, sched, n_iter, elapsed))
     return out
+
+
+def _tc_kernel_name(stats, tiles):
+    """Name of the batch sweep kernel the run used (shared exponentials or not, tile)."""
+    if not stats.get('shared_group'):
+        return 'batch_sweep_kernel'
+    t = (tiles or {}).get('tile')
+    if t and t['rows'] > 2:
+        return 'batch_shared_lds_kernel (%d rows x %d threads)' % (t['rows'], t['threads'])
+    return 'batch_shared_kernel'
 
 
 def _fit_and_sum(samples, pair_ij, n_combo):

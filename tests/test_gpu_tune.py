@@ -112,3 +112,27 @@ def test_autotune_time_course(tune):
     assert set(r.timings) == set(tune.variants(tune.BATCH_SHARED))
     assert tune.get_tile(tune.BATCH_SHARED, r.buckets[0]) == r.best
     tune.reset()
+
+
+def test_every_shared_variant_repairs_fallback_tiles(tune):
+    """An outlier source cell with eps = eps0 = 0.01 makes its tile's fixed-shift
+    sums underflow: with every tile shape of the shared-exponential kernel
+    (register and LDS-staged, 1 to 16 rows per tile) the repair kernel must
+    recompute those tiles and the plans must match the oracle."""
+    from wot_amd.ot import batch
+    coords, growth = _course(3, [120, 140])
+    coords[0] = coords[0].copy()
+    coords[0][0] += 8.0
+    C = O.default_cost_matrix(coords[0], coords[1])
+    lams = (0.5, 1.0, 5.0)
+    refs = [O.transport_stablev2(C, l, 50.0, 0.01, G=growth[0], epsilon0=0.01, **SCHED) for l in lams]
+    for tile in tune.variants(tune.BATCH_SHARED):
+        tune.set_tile(tune.BATCH_SHARED, -1, tile)
+        res = batch.solve_time_course(coords, growth, epsilons=(0.01,), lambda1s=lams, epsilon0=0.01,
+                                      graph_iters=0, **SCHED)
+        assert res.stats['shared_group'] == 3 and res.stats['exact_tiles'] > 0, tile
+        for e, Rref in enumerate(refs):
+            R = res.plan(0, e)
+            err = np.linalg.norm(R - Rref) / np.linalg.norm(Rref)
+            assert err <= 1e-5, (tile, e, err)
+    tune.reset()

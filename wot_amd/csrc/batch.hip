@@ -403,6 +403,194 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
     }
 }
 
+// LDS-staged shared-exponential sweep: a block of NWAVES waves takes a tile of
+// NWAVES * WROWS rows, each wave owning WROWS of them; the other side's
+// reference potentials and column factors are staged in LDS in chunks of
+// 256 columns (double-buffered, loaded once per block instead of once per
+// wave), so their L2 traffic is divided by the rows per tile.  Arithmetic,
+// range checks and epilogue are those of batch_shared_kernel.
+// Columns staged per chunk (multiple of 256) and LDS buffers (2: two barriers
+// per chunk; 3: one).  Measured on the 38-pair, 3 x 3 grid time course with
+// 4 waves x 2 rows: 256/2 6.39 ms, 256/3 6.34, 512/2 5.27, 512/3 5.23 per sweep.
+constexpr int kLdsChunk = 512;
+constexpr int kLdsBuffers = 3;
+
+template <int NG, int NL, int WROWS, int NWAVES, int CH = kLdsChunk, int NBUF = kLdsBuffers>
+__global__ __launch_bounds__(NWAVES * 64) void batch_shared_lds_kernel(BatchArgs ba) {
+    constexpr int E = NG * NL, L0 = NL / 2, THREADS = NWAVES * 64, RE = WROWS * E;
+    constexpr int NB = NL - 1;                        // factor columns per group
+    constexpr int PER = (CH + THREADS - 1) / THREADS;  // staged columns per thread
+    static_assert(RE <= 64, "one lane per (row, combination) of a wave in the epilogue");
+    static_assert(CH % 256 == 0 && (NBUF == 2 || NBUF == 3), "chunk of 64 lanes x 4 columns, 2 or 3 buffers");
+    __shared__ double h_lds[NBUF][NG][CH];
+    __shared__ float b_lds[NBUF][NG * NB][CH];
+    __shared__ int tile_sh;
+    __shared__ int bad_sh;
+    __shared__ double pot_sh[NWAVES][RE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int it = *ba.iter;
+    const IterParam *prm = ba.table + (size_t)it * E;
+    const IterParam *nxt = ba.table + (size_t)(it + ba.k_next) * E;
+    double kg[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) kg[g] = prm[g * NL + L0].k;
+
+    for (;;) {
+        if (tid == 0) tile_sh = atomicAdd(ba.queue, 1);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(tile_sh);
+        if (tid == 0) bad_sh = 0;
+        __syncthreads();
+        if (t >= ba.ntiles) break;
+        const Tile tl = ba.tiles[t];
+        const PairDev pd = ba.pairs[tl.side_pair];
+        const float *A = ba.cost + pd.a_off;
+        const double *h = ba.other + pd.other_off;
+        const float *B = ba.fac_other + pd.other_off;
+        double *pot = ba.pot + pd.pot_off;
+        const int ncols4 = (pd.ncols + 3) & ~3;  // potentials / factors are padded to here
+        const int nchunks = (ncols4 + CH - 1) / CH;
+        const float *rowp[WROWS];
+        double fk[WROWS][NG];
+        float s[WROWS][E];
+        bool far = false;
+#pragma unroll
+        for (int r = 0; r < WROWS; ++r) {
+            const int row = min(tl.row0 + wave * WROWS + r, pd.nrows - 1);
+            rowp[r] = A + (long long)row * pd.lda;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                fk[r][g] = pot[(long long)(g * NL + L0) * pd.pot_ld + row] * kg[g];
+#pragma unroll
+                for (int l = 0; l < NL; ++l) {
+                    const double d = pot[(long long)(g * NL + l) * pd.pot_ld + row] * kg[g] - fk[r][g];
+                    far |= !(fabs(d) <= 64.0);
+                    s[r][g * NL + l] = 0.f;
+                }
+            }
+        }
+        // staging: thread t holds columns t + p * THREADS of a chunk (zero-mass
+        // padding beyond ncols4); loaded into registers one chunk ahead
+        double hv[NG][PER];
+        float bv[NG * NB][PER];
+#define WOTK_LDS_FETCH(k)                                                                        \
+    _Pragma("unroll") for (int p = 0; p < PER; ++p) {                                           \
+        const int j_ = tid + p * THREADS;                                                       \
+        const int col_ = (k) * CH + j_;                                                         \
+        const bool ok_ = j_ < CH && col_ < ncols4;                                              \
+        _Pragma("unroll") for (int g = 0; g < NG; ++g) {                                        \
+            hv[g][p] = ok_ ? h[(long long)(g * NL + L0) * pd.other_ld + col_] : -INFINITY;      \
+            _Pragma("unroll") for (int i = 0; i < NB; ++i) {                                    \
+                const int l_ = i < L0 ? i : i + 1;                                              \
+                bv[g * NB + i][p] = ok_ ? B[(long long)(g * NL + l_) * pd.other_ld + col_] : 0.f; \
+            }                                                                                   \
+        }                                                                                       \
+    }
+#define WOTK_LDS_COMMIT(buf)                                                                     \
+    _Pragma("unroll") for (int p = 0; p < PER; ++p) {                                           \
+        const int j_ = tid + p * THREADS;                                                       \
+        if (j_ < CH) {                                                                          \
+            _Pragma("unroll") for (int g = 0; g < NG; ++g) h_lds[buf][g][j_] = hv[g][p];        \
+            _Pragma("unroll") for (int x = 0; x < NG * NB; ++x) b_lds[buf][x][j_] = bv[x][p];   \
+        }                                                                                       \
+    }
+        WOTK_LDS_FETCH(0)
+        WOTK_LDS_COMMIT(0)
+        __syncthreads();
+        for (int k = 0; k < nchunks; ++k) {
+            const int buf = k % NBUF;
+            if (k + 1 < nchunks) {  // in flight while this chunk computes
+                WOTK_LDS_FETCH(k + 1)
+            }
+#pragma unroll
+            for (int sub = 0; sub < CH / 256; ++sub) {
+            const int cl = sub * 256 + lane * 4;  // column within the chunk
+            const int c = k * CH + cl;
+            if (c < ncols4) {
+                double cv[WROWS][4];
+#pragma unroll
+                for (int r = 0; r < WROWS; ++r) {
+                    const float4 a = bload4(rowp[r] + c);
+                    cv[r][0] = a.x;
+                    cv[r][1] = a.y;
+                    cv[r][2] = a.z;
+                    cv[r][3] = a.w;
+                }
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    const double2 h01 = *reinterpret_cast<const double2 *>(&h_lds[buf][g][cl]);
+                    const double2 h23 = *reinterpret_cast<const double2 *>(&h_lds[buf][g][cl + 2]);
+                    const double hk[4] = {h01.x * kg[g], h01.y * kg[g], h23.x * kg[g], h23.y * kg[g]};
+                    float4 bf[NB];
+#pragma unroll
+                    for (int i = 0; i < NB; ++i) bf[i] = *reinterpret_cast<const float4 *>(&b_lds[buf][g * NB + i][cl]);
+#pragma unroll
+                    for (int r = 0; r < WROWS; ++r) {
+                        float tq[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) tq[q] = bexp2((float)fma(-kg[g], cv[r][q], hk[q] + fk[r][g]));
+                        s[r][g * NL + L0] += (tq[0] + tq[1]) + (tq[2] + tq[3]);
+                        int i = 0;
+#pragma unroll
+                        for (int l = 0; l < NL; ++l)
+                            if (l != L0) {
+                                s[r][g * NL + l] += fmaf(tq[0], bf[i].x, fmaf(tq[1], bf[i].y, fmaf(tq[2], bf[i].z, tq[3] * bf[i].w)));
+                                ++i;
+                            }
+                    }
+                }
+            }
+            }
+            // with two buffers, every wave must be done with chunk k - 1 (same buffer
+            // as k + 1) before it is overwritten; with three, the barrier that ended
+            // iteration k - 1 already ordered that
+            if (NBUF == 2) __syncthreads();
+            if (k + 1 < nchunks) {
+                WOTK_LDS_COMMIT((k + 1) % NBUF)
+            }
+            __syncthreads();
+        }
+        // wave sums (FP32 tree over 64 lanes); lane l < RE takes (row l / E, combination l % E)
+        double S = 0.0, fkl = 0.0;
+#pragma unroll
+        for (int r = 0; r < WROWS; ++r)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                float v = s[r][e];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (lane == r * E + e) {
+                    S = (double)v;
+                    fkl = fk[r][e / NL];
+                }
+            }
+        const int my_r = lane / E, my_e = lane % E;
+        const int my_row = tl.row0 + wave * WROWS + my_r;
+        const bool mine = lane < RE && my_row < pd.nrows;
+        if (far) bad_sh = 1;  // benign race: every writer stores 1
+        if (mine && !(S > 0.0 && S < INFINITY)) atomicOr(&bad_sh, 1);
+        __syncthreads();
+        const int bad = __builtin_amdgcn_readfirstlane(bad_sh);
+        if (bad && tid == 0) ba.bad_list[atomicAdd(ba.n_bad, 1)] = t;
+        if (!bad && lane < RE) {
+            const double p =
+                new_potential(ba, pd, prm[my_e], min(my_row, pd.nrows - 1), tl.side_pair, log2(S) - fkl);
+            pot_sh[wave][lane] = p;
+            if (mine) pot[(long long)my_e * pd.pot_ld + my_row] = p;
+        }
+        __syncthreads();
+        if (!bad && mine) {
+            const int ref = (my_e / NL) * NL + L0;
+            ba.fac[pd.pot_off + (long long)my_e * pd.pot_ld + my_row] =
+                (float)exp2(nxt[ref].k * (pot_sh[wave][lane] - pot_sh[wave][my_r * E + ref]));
+        }
+        __syncthreads();
+    }
+}
+
+#undef WOTK_LDS_FETCH
+#undef WOTK_LDS_COMMIT
+
 // Exact recomputation (per-lane online maximum) of the tiles a sweep queued,
 // row by row, with the sweep's epilogue: potentials and, for the
 // shared-exponential kernel, the column factors.  Runs after every sweep; with
@@ -634,13 +822,26 @@ static const BatchVariant *pick(int E) {
 }
 
 // shared-exponential launchers per (groups, group size, tile variant of kBatchSharedTiles)
+// the kernel of variant V: register-only (wave_rows == 0) or LDS-staged
+template <int NG, int NL, int V>
+static constexpr void (*shared_kernel_of())(BatchArgs) {
+    constexpr TileShape ts = kBatchSharedTiles[V];
+    if constexpr (ts.wave_rows == 0)
+        return batch_shared_kernel<NG, NL, ts.rows, ts.threads>;
+    else
+        return batch_shared_lds_kernel<NG, NL, ts.wave_rows, ts.threads / 64>;
+}
+
 template <int NG, int NL, int V>
 static int launch_shared(const BatchArgs &af, const BatchArgs &ag, int grid_f, int grid_g, int grid_r, int *iter,
                          int *queues, hipStream_t s) {
-    constexpr int R = kBatchSharedTiles[V].rows, T = kBatchSharedTiles[V].threads;
-    batch_shared_kernel<NG, NL, R, T><<<grid_f, T, 0, s>>>(af);
+    constexpr int T = kBatchSharedTiles[V].threads;
+    static_assert(kBatchSharedTiles[V].wave_rows == 0 || kBatchSharedTiles[V].rows == kBatchSharedTiles[V].wave_rows * T / 64,
+                  "LDS variants: rows = waves x rows per wave");
+    constexpr auto kern = shared_kernel_of<NG, NL, V>();
+    kern<<<grid_f, T, 0, s>>>(af);
     batch_repair_kernel<NG * NL, kRepairThreads><<<grid_r, kRepairThreads, 0, s>>>(af);
-    batch_shared_kernel<NG, NL, R, T><<<grid_g, T, 0, s>>>(ag);
+    kern<<<grid_g, T, 0, s>>>(ag);
     batch_repair_kernel<NG * NL, kRepairThreads><<<grid_r, kRepairThreads, 0, s>>>(ag);
     batch_advance_kernel<<<1, 1, 0, s>>>(iter, queues);
     WOTK_LAUNCH_CHECK();
@@ -649,9 +850,9 @@ static int launch_shared(const BatchArgs &af, const BatchArgs &ag, int grid_f, i
 
 template <int NG, int NL, int V>
 static int occupancy_shared() {
-    constexpr int R = kBatchSharedTiles[V].rows, T = kBatchSharedTiles[V].threads;
+    constexpr int T = kBatchSharedTiles[V].threads;
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_shared_kernel<NG, NL, R, T>, T, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, shared_kernel_of<NG, NL, V>(), T, 0) != hipSuccess)
         return 1;
     return per < 1 ? 1 : per;
 }

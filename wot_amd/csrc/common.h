@@ -44,13 +44,15 @@ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 // from these lists, so a variant exists exactly when it is listed here.
 struct TileShape {
     int rows, threads;
+    int wave_rows = 0;  // > 0: LDS-staged variant, each wave owns wave_rows of the tile's rows
 };
 constexpr TileShape kRowSweepTiles[] = {{4, 256}, {1, 256}, {2, 256}, {8, 256},
                                         {2, 512}, {4, 512}, {8, 512}, {4, 128}};
 constexpr int kNumRowSweepTiles = sizeof(kRowSweepTiles) / sizeof(kRowSweepTiles[0]);
 constexpr TileShape kBatchSweepTiles[] = {{2, 256}, {1, 256}, {4, 256}, {1, 512}, {2, 512}};
 constexpr int kNumBatchSweepTiles = sizeof(kBatchSweepTiles) / sizeof(kBatchSweepTiles[0]);
-constexpr TileShape kBatchSharedTiles[] = {{2, 256}, {1, 256}, {4, 256}, {2, 512}};
+constexpr TileShape kBatchSharedTiles[] = {{8, 256, 2},  {4, 256, 1},  {16, 256, 4}, {16, 512, 2},
+                                           {8, 512, 1},  {2, 256},     {1, 256},     {2, 512}};
 constexpr int kNumBatchSharedTiles = sizeof(kBatchSharedTiles) / sizeof(kBatchSharedTiles[0]);
 
 // variant index (into the lists above) a launch of this shape uses
