@@ -246,6 +246,8 @@ def bench_timecourse(args):
                    'tiles': tiles},
         'sinkhorn_iters_per_s': n_solves * n_iter / elapsed,
         'sweep_loop_s': sweep_s,
+        'shared_exponential_group': res.stats.get('shared_group'),
+        'rows_recomputed_exactly': res.stats.get('exact_rows'),
         'roofline': {'bound': 'valu/exp', 'kernel': _tc_kernel_name(res.stats, tiles), 'avg_launch_us': avg_sweep * 1e6,
                      'hbm_achieved': alg_bytes / avg_sweep / 1e9, 'hbm_peak': HBM_PEAK_GBS,
                      'exp_per_s': n_combo * sum_ij / avg_sweep, 'unit': 'GB/s',

@@ -150,12 +150,14 @@ int wotk_plan(const float *C, int64_t ldc, int64_t I, int64_t J, const double *f
  * flags & WOTK_BATCH_SHARE_EXP: combinations are taken in consecutive groups
  * of equal epsilon (n_combos = groups x group size, e.g. an epsilon x lambda1
  * grid in epsilon-major order); within a group one exponential per cost entry
- * serves every combination (exp2(y_ref) times a per-column factor), tiles
- * whose factors leave a safe range are recomputed exactly.  Ignored (the
- * per-combination kernel runs) when the grid has no such grouping.
- * stats: sweeps / sweep_ms as for the single solver; stabilisations = tiles
- * recomputed exactly (fixed shift out of range); gap_checks = combinations per
- * shared-exponential group (0: one exponential per combination).
+ * serves every combination (exp2(y_ref) times a per-column factor); rows
+ * whose factored sums could have lost mass to FP32 underflow, or overflowed,
+ * are recomputed exactly.  Ignored (the per-combination kernel runs) when the
+ * grid has no such grouping.
+ * stats: sweeps / sweep_ms as for the single solver; stabilisations = rows
+ * recomputed exactly (fixed shift or factors out of range); gap_checks =
+ * combinations per shared-exponential group (0: one exponential per
+ * combination).
  * ------------------------------------------------------------------------- */
 #define WOTK_BATCH_SHARE_EXP 1
 

@@ -130,7 +130,7 @@ def test_every_shared_variant_repairs_fallback_tiles(tune):
         tune.set_tile(tune.BATCH_SHARED, -1, tile)
         res = batch.solve_time_course(coords, growth, epsilons=(0.01,), lambda1s=lams, epsilon0=0.01,
                                       graph_iters=0, **SCHED)
-        assert res.stats['shared_group'] == 3 and res.stats['exact_tiles'] > 0, tile
+        assert res.stats['shared_group'] == 3 and res.stats['exact_rows'] > 0, tile
         for e, Rref in enumerate(refs):
             R = res.plan(0, e)
             err = np.linalg.norm(R - Rref) / np.linalg.norm(Rref)

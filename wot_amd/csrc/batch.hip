@@ -70,14 +70,51 @@ struct BatchArgs {
     float *fac;                // shared kernel: this side's column factors ([E][pot_ld], like pot)
     const float *fac_other;    // shared kernel: the other side's column factors
     int k_next;                // shared kernel: factors are read at iteration it + k_next
-    unsigned long long *exact_tiles;  // count of tiles recomputed exactly
-    int *bad_list;             // tiles left to the repair kernel (indices into tiles)
+    unsigned long long *exact_rows;  // count of rows recomputed exactly
+    int *bad_list;             // rows left to the repair kernel: (side_pair, row) pairs
     int *n_bad;                // their count (reset every iteration)
-    int tile_rows;             // rows per tile
-    int group;                 // shared kernel: combinations per group (factors written); 0 otherwise
+    int group;                 // shared kernels: combinations per group (factors written); 0 otherwise
+    float *fmax;               // shared kernels: max(0, factor exponent) per (pair, combination) of this side
+    const float *fmax_other;   // the same for the other side's factors (read)
+    float *fmax_reset;         // cleared by this side's repair launch (the array this sweep consumed)
+    int n_fmax;                // entries of each fmax array (pairs x combinations)
 };
 
+// queue row `row` of side-pair `sp` for the repair kernel
+__device__ __forceinline__ void queue_row(const BatchArgs &ba, int sp, int row) {
+    const int i = atomicAdd(ba.n_bad, 1);
+    ba.bad_list[2 * i] = sp;
+    ba.bad_list[2 * i + 1] = row;
+}
+
+// max of non-negative floats through their integer bit patterns
+__device__ __forceinline__ void atomic_max_nonneg(float *p, float v) {
+    atomicMax(reinterpret_cast<int *>(p), __float_as_int(v));
+}
+
+// factor of combination e at a row from its new potential and the reference's
+// (exponent read by the other side's next sweep with k = kn), and its
+// non-negative exponent for the pair's running maximum (inf when not finite)
+__device__ __forceinline__ float factor_of(double kn, double pot_e, double pot_ref, float *dpos) {
+    const double d = kn * (pot_e - pot_ref);
+    *dpos = d <= 0.0 ? 0.f : (d < 1.0e30 ? (float)d : 1.0e30f);
+    return (float)exp2(d);
+}
+
 __device__ __forceinline__ float bexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// 2^y for an FP64 exponent y, exact up to the FP32 exp2 of a fraction: y = n + r
+// with n = rint(y) and |r| <= 1/2, so the rounding of the exponent to FP32 costs
+// at most 2^-25 relative whatever |y| is (rounding y itself to FP32 costs
+// |y| 2^-24 ln 2).  y is clamped below at -16384 (2^y is 0 in FP32 anyway; the
+// clamp keeps the -inf of padding columns from giving NaN).  Above, y stays
+// below a few hundred for finite potentials; 2^y overflows to inf from 128 on,
+// which the row checks catch.
+__device__ __forceinline__ float bexp2_split(double y) {
+    y = fmax(y, -16384.0);
+    const double n = rint(y);
+    return ldexpf(bexp2((float)(y - n)), (int)n);
+}
 
 __device__ __forceinline__ float4 bload4(const float *p) {
     float4 v;
@@ -188,7 +225,7 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
     constexpr int RE = ROWS * E;
     static_assert(RE <= THREADS, "one thread per (row, combination) in the epilogue");
     __shared__ int tile_sh;
-    __shared__ int bad_sh;
+    __shared__ int bad_row[ROWS];
     __shared__ double red_a[RE][NW];
     __shared__ double l2_sh[RE];  // log2 sum_j 2^((h_j - C_ij) k) per (row, combination)
     const int tid = threadIdx.x;
@@ -198,12 +235,15 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
 #pragma unroll
     for (int e = 0; e < E; ++e) kk[e] = prm[e].k;
 
+    // Every iteration ends on a barrier: lane-specific work sits right after the
+    // barrier at the top, never before the back-edge (where the compiler may
+    // merge it with the tile fetch into a loop nest that parks those lanes
+    // outside the barriers).
     for (;;) {
         if (tid == 0) tile_sh = atomicAdd(ba.queue, 1);
         __syncthreads();
         const int t = __builtin_amdgcn_readfirstlane(tile_sh);
-        if (tid == 0) bad_sh = 0;
-        __syncthreads();
+        if (tid < ROWS) bad_row[tid] = 0;  // last read before the previous iteration's final barrier
         if (t >= ba.ntiles) break;
         const Tile tl = ba.tiles[t];
         const PairDev pd = ba.pairs[tl.side_pair];
@@ -257,22 +297,35 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
             for (int w = 0; w < NW; ++w) S += red_a[tid][w];
             const int r = tid / E, e = tid % E;
             l2_sh[tid] = log2(S) - fk[r][e];
-            if (!(S > 0.0 && S < INFINITY) && tl.row0 + r < pd.nrows) atomicOr(&bad_sh, 1);
+            if (!(S > 0.0 && S < INFINITY) && tl.row0 + r < pd.nrows) atomicOr(&bad_row[r], 1);
         }
         __syncthreads();
         // rare: the fixed shift under- or overflowed a row; the repair kernel
-        // recomputes the tile exactly.  The flag is read as a scalar and no
-        // barrier sits under a condition, so every lane keeps the same path.
-        const int bad = __builtin_amdgcn_readfirstlane(bad_sh);
-        if (bad && tid == 0) ba.bad_list[atomicAdd(ba.n_bad, 1)] = t;
-        if (!bad && tid < RE) {
+        // recomputes that row exactly.  No barrier sits under a condition.
+        if (tid < RE) {
             const int r = tid / E, e = tid % E;
             const int row = tl.row0 + r;
-            if (row < pd.nrows)
-                pot[(long long)e * pd.pot_ld + row] = new_potential(ba, pd, prm[e], row, tl.side_pair, l2_sh[tid]);
+            if (row < pd.nrows) {
+                if (!bad_row[r])
+                    pot[(long long)e * pd.pot_ld + row] = new_potential(ba, pd, prm[e], row, tl.side_pair, l2_sh[tid]);
+                else if (e == 0)
+                    queue_row(ba, tl.side_pair, row);
+            }
         }
         __syncthreads();
     }
+}
+
+// Is a factored row sum S usable?  The terms are 2^(y_ref) * B_j with 2^(y_ref)
+// from bexp2_split (exact to 2^-25 for any offset between the combinations); a
+// term whose 2^(y_ref) underflows FP32 (below 2^-126, flushed or denormal) is
+// lost, and it was worth less than 2^-126 * B_j <= 2^(-126 + dmax) with dmax
+// the largest factor exponent of the pair's columns, so the lost mass of a row
+// of ncols terms is below 2^(log2 ncols + dmax - 126).  S must exceed that by
+// 2^30 and be finite.
+__device__ __forceinline__ bool factored_sum_ok(double S, int ncols, float dmax) {
+    const double lo = exp2(log2((double)ncols) + (double)dmax - 96.0);
+    return S >= lo && S < INFINITY;
 }
 
 // Shared-exponential sweep for NG groups of NL combinations of equal epsilon
@@ -284,14 +337,15 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
 // epilogue), so that
 //   log2 sum_j 2^((h^e_j - C_ij) k) = log2 sum_j 2^y_ij B^e_j - k f^ref_i,old.
 // The terms are combination e's previous plan entries scaled by
-// 2^(-k (f^e_i - f^ref_i)); a tile whose row offsets exceed 2^64 either way, or
-// whose sums under- or overflow (B = inf or 0 included), is recomputed exactly.
+// 2^(-k (f^e_i - f^ref_i)); a row whose sums leave the range that
+// factored_sum_ok accepts is recomputed exactly by the repair kernel.
 template <int NG, int NL, int ROWS, int THREADS>
 __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
     constexpr int E = NG * NL, L0 = NL / 2, NW = THREADS / 64, RE = ROWS * E;
     static_assert(RE <= THREADS, "one thread per (row, combination) in the epilogue");
     __shared__ int tile_sh;
-    __shared__ int bad_sh;
+    __shared__ int bad_row[ROWS];
+    __shared__ float fmax_sh[E];
     __shared__ double red_a[RE][NW];
     __shared__ double l2_sh[RE];
     __shared__ double pot_sh[RE];
@@ -302,16 +356,23 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
     double kg[NG];
 #pragma unroll
     for (int g = 0; g < NG; ++g) kg[g] = prm[g * NL + L0].k;
+    if (tid < E) fmax_sh[tid] = 0.f;
+    int prev_pair = -1;  // pair of the previous tile, whose factor maxima are flushed at the top
 
+    // every iteration ends on a barrier (see batch_sweep_kernel)
     for (;;) {
         if (tid == 0) tile_sh = atomicAdd(ba.queue, 1);
         __syncthreads();
         const int t = __builtin_amdgcn_readfirstlane(tile_sh);
-        if (tid == 0) bad_sh = 0;
-        __syncthreads();
+        if (tid < ROWS) bad_row[tid] = 0;
+        if (tid < E && prev_pair >= 0) {
+            atomic_max_nonneg(&ba.fmax[prev_pair * E + tid], fmax_sh[tid]);
+            fmax_sh[tid] = 0.f;
+        }
         if (t >= ba.ntiles) break;
         const Tile tl = ba.tiles[t];
         const PairDev pd = ba.pairs[tl.side_pair];
+        prev_pair = pd.pair;
         const float *A = ba.cost + pd.a_off;
         const double *h = ba.other + pd.other_off;
         const float *B = ba.fac_other + pd.other_off;
@@ -319,7 +380,6 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
         const float *rowp[ROWS];
         double fk[ROWS][NG];
         float s[ROWS][E];
-        bool far = false;
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
             const int row = min(tl.row0 + r, pd.nrows - 1);
@@ -328,11 +388,7 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
             for (int g = 0; g < NG; ++g) {
                 fk[r][g] = pot[(long long)(g * NL + L0) * pd.pot_ld + row] * kg[g];
 #pragma unroll
-                for (int l = 0; l < NL; ++l) {
-                    const double d = pot[(long long)(g * NL + l) * pd.pot_ld + row] * kg[g] - fk[r][g];
-                    far |= !(fabs(d) <= 64.0);
-                    s[r][g * NL + l] = 0.f;
-                }
+                for (int l = 0; l < NL; ++l) s[r][g * NL + l] = 0.f;
             }
         }
         for (int c = tid * 4; c < pd.ncols; c += THREADS * 4) {
@@ -359,7 +415,7 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
                 for (int r = 0; r < ROWS; ++r) {
                     float tq[4];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) tq[q] = bexp2((float)fma(-kg[g], cv[r][q], hk[q] + fk[r][g]));
+                    for (int q = 0; q < 4; ++q) tq[q] = bexp2_split(fma(-kg[g], cv[r][q], hk[q] + fk[r][g]));
                     s[r][g * NL + L0] += (tq[0] + tq[1]) + (tq[2] + tq[3]);
 #pragma unroll
                     for (int l = 0; l < NL; ++l)
@@ -369,35 +425,43 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
             }
         }
         lane_sums<RE, NW>(&s[0][0], red_a);
-        if (far) bad_sh = 1;  // benign race: every writer stores 1
         __syncthreads();
         if (tid < RE) {
             double S = 0.0;
 #pragma unroll
             for (int w = 0; w < NW; ++w) S += red_a[tid][w];
-            const int r = tid / E, g = (tid % E) / NL;
+            const int r = tid / E, e = tid % E, g = e / NL;
+            const int row = tl.row0 + r;
             l2_sh[tid] = log2(S) - fk[r][g];
-            if (!(S > 0.0 && S < INFINITY) && tl.row0 + r < pd.nrows) atomicOr(&bad_sh, 1);
+            if (row < pd.nrows &&
+                !factored_sum_ok(S, pd.ncols, ba.fmax_other[pd.pair * E + e]))
+                atomicOr(&bad_row[r], 1);
         }
         __syncthreads();
-        // the flag is read as a scalar and no barrier sits under a condition
-        const int bad = __builtin_amdgcn_readfirstlane(bad_sh);
-        if (bad && tid == 0) ba.bad_list[atomicAdd(ba.n_bad, 1)] = t;
-        if (!bad && tid < RE) {
+        // rows out of range go to the repair kernel; no barrier sits under a condition
+        if (tid < RE) {
             const int r = tid / E, e = tid % E;
             const int row = tl.row0 + r;
             const double p = new_potential(ba, pd, prm[e], min(row, pd.nrows - 1), tl.side_pair, l2_sh[tid]);
             pot_sh[tid] = p;
-            if (row < pd.nrows) pot[(long long)e * pd.pot_ld + row] = p;
+            if (row < pd.nrows) {
+                if (!bad_row[r])
+                    pot[(long long)e * pd.pot_ld + row] = p;
+                else if (e == 0)
+                    queue_row(ba, tl.side_pair, row);
+            }
         }
         __syncthreads();
-        if (!bad && tid < RE) {
+        if (tid < RE) {
             // this side's factors, read by the other side's next sweep with k of that sweep
             const int r = tid / E, e = tid % E, g = e / NL;
             const int row = tl.row0 + r;
-            if (row < pd.nrows)
+            if (row < pd.nrows && !bad_row[r]) {
+                float dpos;
                 ba.fac[pd.pot_off + (long long)e * pd.pot_ld + row] =
-                    (float)exp2(nxt[g * NL + L0].k * (pot_sh[tid] - pot_sh[r * E + g * NL + L0]));
+                    factor_of(nxt[g * NL + L0].k, pot_sh[tid], pot_sh[r * E + g * NL + L0], &dpos);
+                atomic_max_nonneg(&fmax_sh[e], dpos);
+            }
         }
         __syncthreads();
     }
@@ -410,13 +474,17 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
 // wave), so their L2 traffic is divided by the rows per tile.  Arithmetic,
 // range checks and epilogue are those of batch_shared_kernel.
 // Columns staged per chunk (multiple of 256) and LDS buffers (2: two barriers
-// per chunk; 3: one).  Measured on the 38-pair, 3 x 3 grid time course with
-// 4 waves x 2 rows: 256/2 6.39 ms, 256/3 6.34, 512/2 5.27, 512/3 5.23 per sweep.
-constexpr int kLdsChunk = 512;
+// per chunk; 3: one), and the kernel is compiled for 3 waves per SIMD (the
+// register cap spills a few values the chunk loop does not touch).  Measured on
+// the 38-pair, 3 x 3 grid time course, ms per sweep with 4 waves x 2 rows /
+// 4 waves x 4 rows: 512/3 at 2 waves per SIMD 5.32 / 10.6; 512/2 at 3 waves
+// 5.26 / 5.88; 256/3 at 3 waves 5.18 / 5.11; 256/2 at 3 waves 5.92 / 5.11.
+constexpr int kLdsChunk = 256;
 constexpr int kLdsBuffers = 3;
 
 template <int NG, int NL, int WROWS, int NWAVES, int CH = kLdsChunk, int NBUF = kLdsBuffers>
-__global__ __launch_bounds__(NWAVES * 64) void batch_shared_lds_kernel(BatchArgs ba) {
+__global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 8))) void batch_shared_lds_kernel(
+    BatchArgs ba) {
     constexpr int E = NG * NL, L0 = NL / 2, THREADS = NWAVES * 64, RE = WROWS * E;
     constexpr int NB = NL - 1;                        // factor columns per group
     constexpr int PER = (CH + THREADS - 1) / THREADS;  // staged columns per thread
@@ -425,7 +493,8 @@ __global__ __launch_bounds__(NWAVES * 64) void batch_shared_lds_kernel(BatchArgs
     __shared__ double h_lds[NBUF][NG][CH];
     __shared__ float b_lds[NBUF][NG * NB][CH];
     __shared__ int tile_sh;
-    __shared__ int bad_sh;
+    __shared__ int bad_row[NWAVES * WROWS];
+    __shared__ float fmax_sh[E];
     __shared__ double pot_sh[NWAVES][RE];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int it = *ba.iter;
@@ -434,16 +503,23 @@ __global__ __launch_bounds__(NWAVES * 64) void batch_shared_lds_kernel(BatchArgs
     double kg[NG];
 #pragma unroll
     for (int g = 0; g < NG; ++g) kg[g] = prm[g * NL + L0].k;
+    if (tid < E) fmax_sh[tid] = 0.f;
+    int prev_pair = -1;  // pair of the previous tile, whose factor maxima are flushed at the top
 
+    // every iteration ends on a barrier (see batch_sweep_kernel)
     for (;;) {
         if (tid == 0) tile_sh = atomicAdd(ba.queue, 1);
         __syncthreads();
         const int t = __builtin_amdgcn_readfirstlane(tile_sh);
-        if (tid == 0) bad_sh = 0;
-        __syncthreads();
+        if (tid < NWAVES * WROWS) bad_row[tid] = 0;
+        if (tid < E && prev_pair >= 0) {
+            atomic_max_nonneg(&ba.fmax[prev_pair * E + tid], fmax_sh[tid]);
+            fmax_sh[tid] = 0.f;
+        }
         if (t >= ba.ntiles) break;
         const Tile tl = ba.tiles[t];
         const PairDev pd = ba.pairs[tl.side_pair];
+        prev_pair = pd.pair;
         const float *A = ba.cost + pd.a_off;
         const double *h = ba.other + pd.other_off;
         const float *B = ba.fac_other + pd.other_off;
@@ -453,7 +529,6 @@ __global__ __launch_bounds__(NWAVES * 64) void batch_shared_lds_kernel(BatchArgs
         const float *rowp[WROWS];
         double fk[WROWS][NG];
         float s[WROWS][E];
-        bool far = false;
 #pragma unroll
         for (int r = 0; r < WROWS; ++r) {
             const int row = min(tl.row0 + wave * WROWS + r, pd.nrows - 1);
@@ -462,11 +537,7 @@ __global__ __launch_bounds__(NWAVES * 64) void batch_shared_lds_kernel(BatchArgs
             for (int g = 0; g < NG; ++g) {
                 fk[r][g] = pot[(long long)(g * NL + L0) * pd.pot_ld + row] * kg[g];
 #pragma unroll
-                for (int l = 0; l < NL; ++l) {
-                    const double d = pot[(long long)(g * NL + l) * pd.pot_ld + row] * kg[g] - fk[r][g];
-                    far |= !(fabs(d) <= 64.0);
-                    s[r][g * NL + l] = 0.f;
-                }
+                for (int l = 0; l < NL; ++l) s[r][g * NL + l] = 0.f;
             }
         }
         // staging: thread t holds columns t + p * THREADS of a chunk (zero-mass
@@ -528,7 +599,7 @@ __global__ __launch_bounds__(NWAVES * 64) void batch_shared_lds_kernel(BatchArgs
                     for (int r = 0; r < WROWS; ++r) {
                         float tq[4];
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) tq[q] = bexp2((float)fma(-kg[g], cv[r][q], hk[q] + fk[r][g]));
+                        for (int q = 0; q < 4; ++q) tq[q] = bexp2_split(fma(-kg[g], cv[r][q], hk[q] + fk[r][g]));
                         s[r][g * NL + L0] += (tq[0] + tq[1]) + (tq[2] + tq[3]);
                         int i = 0;
 #pragma unroll
@@ -567,22 +638,27 @@ __global__ __launch_bounds__(NWAVES * 64) void batch_shared_lds_kernel(BatchArgs
         const int my_r = lane / E, my_e = lane % E;
         const int my_row = tl.row0 + wave * WROWS + my_r;
         const bool mine = lane < RE && my_row < pd.nrows;
-        if (far) bad_sh = 1;  // benign race: every writer stores 1
-        if (mine && !(S > 0.0 && S < INFINITY)) atomicOr(&bad_sh, 1);
+        if (mine && !factored_sum_ok(S, pd.ncols, ba.fmax_other[pd.pair * E + my_e]))
+            atomicOr(&bad_row[wave * WROWS + my_r], 1);
         __syncthreads();
-        const int bad = __builtin_amdgcn_readfirstlane(bad_sh);
-        if (bad && tid == 0) ba.bad_list[atomicAdd(ba.n_bad, 1)] = t;
-        if (!bad && lane < RE) {
+        // rows out of range go to the repair kernel; no barrier sits under a condition
+        const bool row_ok = mine && !bad_row[wave * WROWS + my_r];
+        if (lane < RE) {
             const double p =
                 new_potential(ba, pd, prm[my_e], min(my_row, pd.nrows - 1), tl.side_pair, log2(S) - fkl);
             pot_sh[wave][lane] = p;
-            if (mine) pot[(long long)my_e * pd.pot_ld + my_row] = p;
+            if (row_ok)
+                pot[(long long)my_e * pd.pot_ld + my_row] = p;
+            else if (mine && my_e == 0)
+                queue_row(ba, tl.side_pair, my_row);
         }
         __syncthreads();
-        if (!bad && mine) {
+        if (row_ok) {
             const int ref = (my_e / NL) * NL + L0;
+            float dpos;
             ba.fac[pd.pot_off + (long long)my_e * pd.pot_ld + my_row] =
-                (float)exp2(nxt[ref].k * (pot_sh[wave][lane] - pot_sh[wave][my_r * E + ref]));
+                factor_of(nxt[ref].k, pot_sh[wave][lane], pot_sh[wave][my_r * E + ref], &dpos);
+            atomic_max_nonneg(&fmax_sh[my_e], dpos);
         }
         __syncthreads();
     }
@@ -591,10 +667,11 @@ __global__ __launch_bounds__(NWAVES * 64) void batch_shared_lds_kernel(BatchArgs
 #undef WOTK_LDS_FETCH
 #undef WOTK_LDS_COMMIT
 
-// Exact recomputation (per-lane online maximum) of the tiles a sweep queued,
-// row by row, with the sweep's epilogue: potentials and, for the
-// shared-exponential kernel, the column factors.  Runs after every sweep; with
-// no queued tile it exits at once.
+// Exact recomputation (per-lane online maximum) of the rows a sweep queued,
+// with the sweep's epilogue: potentials and, for the shared-exponential
+// kernels, the column factors and their maximum exponent.  Runs after every
+// sweep; block 0 also clears the factor maxima the sweep just consumed (the
+// other side's, rewritten by the next sweep).  With no queued row it exits.
 template <int E, int THREADS>
 __global__ __launch_bounds__(THREADS) void batch_repair_kernel(BatchArgs ba) {
     constexpr int NW = THREADS / 64;
@@ -603,34 +680,34 @@ __global__ __launch_bounds__(THREADS) void batch_repair_kernel(BatchArgs ba) {
     __shared__ double l2_sh[E];
     __shared__ double pot_sh[E];
     const int tid = threadIdx.x;
+    if (blockIdx.x == 0 && ba.group)
+        for (int i = tid; i < ba.n_fmax; i += THREADS) ba.fmax_reset[i] = 0.f;
     const int nb = __builtin_amdgcn_readfirstlane(*ba.n_bad);
     if ((int)blockIdx.x >= nb) return;
     const int it = *ba.iter;
     const IterParam *prm = ba.table + (size_t)it * E;
     const IterParam *nxt = ba.table + (size_t)(it + ba.k_next) * E;
-    if (blockIdx.x == 0 && tid == 0) atomicAdd(ba.exact_tiles, (unsigned long long)nb);
+    if (blockIdx.x == 0 && tid == 0) atomicAdd(ba.exact_rows, (unsigned long long)nb);
     for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-        const Tile tl = ba.tiles[ba.bad_list[b]];
-        const PairDev pd = ba.pairs[tl.side_pair];
+        const int sp = ba.bad_list[2 * b], row = ba.bad_list[2 * b + 1];
+        const PairDev pd = ba.pairs[sp];
         const float *A = ba.cost + pd.a_off;
         const double *h = ba.other + pd.other_off;
         double *pot = ba.pot + pd.pot_off;
-        const int rend = min(tl.row0 + ba.tile_rows, pd.nrows);
-        for (int row = tl.row0; row < rend; ++row) {
-            row_exact<E, THREADS>(A, pd.lda, row, pd.ncols, h, pd.other_ld, prm, red_a, red_b, l2_sh);
-            if (tid < E) {
-                const double p = new_potential(ba, pd, prm[tid], row, tl.side_pair, l2_sh[tid]);
-                pot_sh[tid] = p;
-                pot[(long long)tid * pd.pot_ld + row] = p;
-            }
-            __syncthreads();
-            if (ba.group && tid < E) {
-                const int ref = (tid / ba.group) * ba.group + ba.group / 2;
-                ba.fac[pd.pot_off + (long long)tid * pd.pot_ld + row] =
-                    (float)exp2(nxt[ref].k * (pot_sh[tid] - pot_sh[ref]));
-            }
-            __syncthreads();
+        row_exact<E, THREADS>(A, pd.lda, row, pd.ncols, h, pd.other_ld, prm, red_a, red_b, l2_sh);
+        if (tid < E) {
+            const double p = new_potential(ba, pd, prm[tid], row, sp, l2_sh[tid]);
+            pot_sh[tid] = p;
+            pot[(long long)tid * pd.pot_ld + row] = p;
         }
+        __syncthreads();
+        if (ba.group && tid < E) {
+            const int ref = (tid / ba.group) * ba.group + ba.group / 2;
+            float dpos;
+            ba.fac[pd.pot_off + (long long)tid * pd.pot_ld + row] = factor_of(nxt[ref].k, pot_sh[tid], pot_sh[ref], &dpos);
+            atomic_max_nonneg(&ba.fmax[pd.pair * E + tid], dpos);
+        }
+        __syncthreads();
     }
 }
 
@@ -643,13 +720,17 @@ __global__ void batch_advance_kernel(int *iter, int *queues) {
     queues[3] = 0;
 }
 
-__global__ void batch_reset_kernel(int *iter, int *queues, unsigned long long *exact_tiles, int start) {
+__global__ void batch_reset_kernel(int *iter, int *queues, unsigned long long *exact_rows, int start) {
     iter[0] = start;
     queues[0] = 0;
     queues[1] = 0;
     queues[2] = 0;
     queues[3] = 0;
-    exact_tiles[0] = 0;
+    exact_rows[0] = 0;
+}
+
+__global__ void batch_zero_f32_kernel(float *x, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = 0.f;
 }
 
 __global__ void batch_fill_kernel(double *x, long long n, double v) {
@@ -710,8 +791,9 @@ struct BatchWork {
     long long *g_off, *lw_off;
     int *n_src;
     int *iter, *queues;
-    unsigned long long *exact_tiles;
-    int *bad_f, *bad_g;  // repair lists
+    unsigned long long *exact_rows;
+    int *bad_f, *bad_g;      // repair lists: (side_pair, row) pairs
+    float *fmax_f, *fmax_g;  // factor maxima per (pair, combination)
 };
 
 }  // namespace wotk
@@ -741,14 +823,16 @@ static size_t batch_carve(Arena &ar, const wotk_batch_pair *pairs, int n_pairs, 
     w->pairs_g = ar.take<PairDev>(n_pairs);
     w->tiles_f = ar.take<Tile>(tf);
     w->tiles_g = ar.take<Tile>(tg);
-    w->bad_f = ar.take<int>(tf);
-    w->bad_g = ar.take<int>(tg);
+    w->bad_f = ar.take<int>(2 * tf);
+    w->bad_g = ar.take<int>(2 * tg);
+    w->fmax_f = ar.take<float>((size_t)n_pairs * E);
+    w->fmax_g = ar.take<float>((size_t)n_pairs * E);
     w->g_off = ar.take<long long>(n_pairs);
     w->lw_off = ar.take<long long>(n_pairs);
     w->n_src = ar.take<int>(n_pairs);
     w->iter = ar.take<int>(4);
     w->queues = ar.take<int>(4);
-    w->exact_tiles = ar.take<unsigned long long>(1);
+    w->exact_rows = ar.take<unsigned long long>(1);
     *nf_out = nf;
     *ng_out = ng;
     *ntf = tf;
@@ -1003,15 +1087,19 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     batch_fac_init_kernel<<<n_pairs, 256, 0, s>>>(w.fg, w.pairs_g, n_pairs, E);
     WOTK_LAUNCH_CHECK();
     batch_growth_kernel<<<n_pairs, 256, 0, s>>>(G, w.g_off, w.n_src, w.logw, w.lw_off, w.logq);
-    batch_reset_kernel<<<1, 1, 0, s>>>(w.iter, w.queues, w.exact_tiles, 0);
+    batch_reset_kernel<<<1, 1, 0, s>>>(w.iter, w.queues, w.exact_rows, 0);
+    batch_zero_f32_kernel<<<1, 256, 0, s>>>(w.fmax_f, n_pairs * E);
+    batch_zero_f32_kernel<<<1, 256, 0, s>>>(w.fmax_g, n_pairs * E);
     WOTK_LAUNCH_CHECK();
     WOTK_HIP(hipStreamSynchronize(s));  // host tables above are released at return
 
     const int64_t n_tf = (int64_t)tf.size(), n_tg = (int64_t)tg.size();  // <= ntf, ntg (one-row tiles)
     BatchArgs af{cost, w.pairs_f, w.tiles_f, (int)n_tf, w.table, w.iter, E, 0, w.f, w.g, w.logw, w.logq, w.lognf,
-                 w.queues, w.ff, w.fg, 0, w.exact_tiles, w.bad_f, w.queues + 2, tile_rows, group};
+                 w.queues, w.ff, w.fg, 0, w.exact_rows, w.bad_f, w.queues + 2, group, w.fmax_f, w.fmax_g,
+                 w.fmax_g, n_pairs * E};
     BatchArgs ag{cost, w.pairs_g, w.tiles_g, (int)n_tg, w.table, w.iter, E, 1, w.g, w.f, w.logw, w.logq, w.logng,
-                 w.queues + 1, w.fg, w.ff, 1, w.exact_tiles, w.bad_g, w.queues + 3, tile_rows, group};
+                 w.queues + 1, w.fg, w.ff, 1, w.exact_rows, w.bad_g, w.queues + 3, group, w.fmax_g, w.fmax_f,
+                 w.fmax_f, n_pairs * E};
     const int grid = persistent_grid(*variant);
     const int grid_f = (int)std::min<int64_t>(grid, n_tf), grid_g = (int)std::min<int64_t>(grid, n_tg);
     const SweepFn sweep = variant->sweep;
@@ -1071,8 +1159,8 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
             ob += (long long)E * pairs[p].n_j;
         }
     }
-    unsigned long long exact_tiles = 0;
-    WOTK_HIP(hipMemcpyAsync(&exact_tiles, w.exact_tiles, sizeof(exact_tiles), hipMemcpyDeviceToHost, s));
+    unsigned long long exact_rows = 0;
+    WOTK_HIP(hipMemcpyAsync(&exact_rows, w.exact_rows, sizeof(exact_rows), hipMemcpyDeviceToHost, s));
     WOTK_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
     WOTK_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
@@ -1087,7 +1175,7 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
         stats->sweeps = 2 * n_iter;
         stats->sweep_ms = ms;
         stats->epsilon_final = table[(size_t)n_iter * E].eps;
-        stats->stabilisations = (int64_t)exact_tiles;
+        stats->stabilisations = (int64_t)exact_rows;
         stats->gap_checks = group;
     }
     return WOTK_OK;

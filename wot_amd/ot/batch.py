@@ -183,7 +183,7 @@ def solve_prepared(tc, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_
             a += Eg * n_i
             b += Eg * n_j
         d = st.as_dict()
-        d['exact_tiles'] = d['stabilisations']  # tiles recomputed with the online maximum
+        d['exact_rows'] = d['stabilisations']  # rows recomputed with the online maximum
         d['shared_group'] = d['gap_checks']     # combinations per shared exponential (0: none)
         if st_total is None:
             st_total = d
@@ -191,7 +191,7 @@ def solve_prepared(tc, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_
             st_total['sweep_ms'] += d['sweep_ms']
             st_total['sweeps'] += d['sweeps']
             st_total['stabilisations'] += d['stabilisations']
-            st_total['exact_tiles'] += d['exact_tiles']
+            st_total['exact_rows'] += d['exact_rows']
 
     def reg(e, n):
         return (epsilon0 - e) * np.exp(-n) + e
