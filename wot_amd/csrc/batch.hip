@@ -71,7 +71,7 @@ struct BatchArgs {
     const float *fac_other;    // shared kernel: the other side's column factors
     int k_next;                // shared kernel: factors are read at iteration it + k_next
     unsigned long long *exact_rows;  // count of rows recomputed exactly
-    int *bad_list;             // rows left to the repair kernel: (side_pair, row) pairs
+    int *bad_list;             // rows left to the repair kernel: (side_pair, row, combination mask)
     int *n_bad;                // their count (reset every iteration)
     int group;                 // shared kernels: combinations per group (factors written); 0 otherwise
     float *fmax;               // shared kernels: max(0, factor exponent) per (pair, combination) of this side
@@ -80,11 +80,24 @@ struct BatchArgs {
     int n_fmax;                // entries of each fmax array (pairs x combinations)
 };
 
-// queue row `row` of side-pair `sp` for the repair kernel
-__device__ __forceinline__ void queue_row(const BatchArgs &ba, int sp, int row) {
+// queue combinations `mask` (bit e: combination e) of row `row` of side-pair
+// `sp` for the repair kernel
+__device__ __forceinline__ void queue_row(const BatchArgs &ba, int sp, int row, int mask) {
     const int i = atomicAdd(ba.n_bad, 1);
-    ba.bad_list[2 * i] = sp;
-    ba.bad_list[2 * i + 1] = row;
+    ba.bad_list[3 * i] = sp;
+    ba.bad_list[3 * i + 1] = row;
+    ba.bad_list[3 * i + 2] = mask;
+}
+
+// a failing reference invalidates every factor of its group: widen the mask to
+// whole groups (NL combinations per group, reference NL / 2)
+template <int NG, int NL>
+__device__ __forceinline__ int widen_to_groups(int m) {
+    int w = m;
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+        if ((m >> (g * NL + NL / 2)) & 1) w |= ((1 << NL) - 1) << (g * NL);
+    return w;
 }
 
 // max of non-negative floats through their integer bit patterns
@@ -133,8 +146,8 @@ __device__ __forceinline__ float4 bload4(const float *p) {
 // prm[e].k.  Every thread of the block calls it (it synchronises).
 template <int E, int THREADS>
 __device__ __forceinline__ void row_exact(const float *A, long long lda, int row, int ncols, const double *h,
-                                          long long other_ld, const IterParam *prm, double *red_a, double *red_b,
-                                          double *l2) {
+                                          long long other_ld, const IterParam *prm, int mask, double *red_a,
+                                          double *red_b, double *l2) {
     constexpr int NW = THREADS / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     {
@@ -151,6 +164,7 @@ __device__ __forceinline__ void row_exact(const float *A, long long lda, int row
             const double av[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
             for (int e = 0; e < E; ++e) {
+                if (!((mask >> e) & 1)) continue;  // block-uniform
                 const double k = prm[e].k;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -297,19 +311,19 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
             for (int w = 0; w < NW; ++w) S += red_a[tid][w];
             const int r = tid / E, e = tid % E;
             l2_sh[tid] = log2(S) - fk[r][e];
-            if (!(S > 0.0 && S < INFINITY) && tl.row0 + r < pd.nrows) atomicOr(&bad_row[r], 1);
+            if (!(S > 0.0 && S < INFINITY) && tl.row0 + r < pd.nrows) atomicOr(&bad_row[r], 1 << e);
         }
         __syncthreads();
-        // rare: the fixed shift under- or overflowed a row; the repair kernel
-        // recomputes that row exactly.  No barrier sits under a condition.
+        // rare: the fixed shift under- or overflowed; the repair kernel recomputes
+        // those (row, combination) exactly.  No barrier sits under a condition.
         if (tid < RE) {
             const int r = tid / E, e = tid % E;
             const int row = tl.row0 + r;
+            const int m = bad_row[r];
             if (row < pd.nrows) {
-                if (!bad_row[r])
+                if (!((m >> e) & 1))
                     pot[(long long)e * pd.pot_ld + row] = new_potential(ba, pd, prm[e], row, tl.side_pair, l2_sh[tid]);
-                else if (e == 0)
-                    queue_row(ba, tl.side_pair, row);
+                if (m && e == 0) queue_row(ba, tl.side_pair, row, m);
             }
         }
         __syncthreads();
@@ -435,20 +449,20 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
             l2_sh[tid] = log2(S) - fk[r][g];
             if (row < pd.nrows &&
                 !factored_sum_ok(S, pd.ncols, ba.fmax_other[pd.pair * E + e]))
-                atomicOr(&bad_row[r], 1);
+                atomicOr(&bad_row[r], 1 << e);
         }
         __syncthreads();
-        // rows out of range go to the repair kernel; no barrier sits under a condition
+        // combinations out of range go to the repair kernel (whole groups when the
+        // reference failed); no barrier sits under a condition
         if (tid < RE) {
             const int r = tid / E, e = tid % E;
             const int row = tl.row0 + r;
+            const int m = widen_to_groups<NG, NL>(bad_row[r]);
             const double p = new_potential(ba, pd, prm[e], min(row, pd.nrows - 1), tl.side_pair, l2_sh[tid]);
             pot_sh[tid] = p;
             if (row < pd.nrows) {
-                if (!bad_row[r])
-                    pot[(long long)e * pd.pot_ld + row] = p;
-                else if (e == 0)
-                    queue_row(ba, tl.side_pair, row);
+                if (!((m >> e) & 1)) pot[(long long)e * pd.pot_ld + row] = p;
+                if (m && e == 0) queue_row(ba, tl.side_pair, row, m);
             }
         }
         __syncthreads();
@@ -456,7 +470,7 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
             // this side's factors, read by the other side's next sweep with k of that sweep
             const int r = tid / E, e = tid % E, g = e / NL;
             const int row = tl.row0 + r;
-            if (row < pd.nrows && !bad_row[r]) {
+            if (row < pd.nrows && !((widen_to_groups<NG, NL>(bad_row[r]) >> e) & 1)) {
                 float dpos;
                 ba.fac[pd.pot_off + (long long)e * pd.pot_ld + row] =
                     factor_of(nxt[g * NL + L0].k, pot_sh[tid], pot_sh[r * E + g * NL + L0], &dpos);
@@ -639,18 +653,18 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 
         const int my_row = tl.row0 + wave * WROWS + my_r;
         const bool mine = lane < RE && my_row < pd.nrows;
         if (mine && !factored_sum_ok(S, pd.ncols, ba.fmax_other[pd.pair * E + my_e]))
-            atomicOr(&bad_row[wave * WROWS + my_r], 1);
+            atomicOr(&bad_row[wave * WROWS + my_r], 1 << my_e);
         __syncthreads();
-        // rows out of range go to the repair kernel; no barrier sits under a condition
-        const bool row_ok = mine && !bad_row[wave * WROWS + my_r];
+        // combinations out of range go to the repair kernel (whole groups when the
+        // reference failed); no barrier sits under a condition
+        const int m = lane < RE ? widen_to_groups<NG, NL>(bad_row[wave * WROWS + my_r]) : 0;
+        const bool row_ok = mine && !((m >> my_e) & 1);
         if (lane < RE) {
             const double p =
                 new_potential(ba, pd, prm[my_e], min(my_row, pd.nrows - 1), tl.side_pair, log2(S) - fkl);
             pot_sh[wave][lane] = p;
-            if (row_ok)
-                pot[(long long)my_e * pd.pot_ld + my_row] = p;
-            else if (mine && my_e == 0)
-                queue_row(ba, tl.side_pair, my_row);
+            if (row_ok) pot[(long long)my_e * pd.pot_ld + my_row] = p;
+            if (mine && m && my_e == 0) queue_row(ba, tl.side_pair, my_row, m);
         }
         __syncthreads();
         if (row_ok) {
@@ -689,19 +703,24 @@ __global__ __launch_bounds__(THREADS) void batch_repair_kernel(BatchArgs ba) {
     const IterParam *nxt = ba.table + (size_t)(it + ba.k_next) * E;
     if (blockIdx.x == 0 && tid == 0) atomicAdd(ba.exact_rows, (unsigned long long)nb);
     for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-        const int sp = ba.bad_list[2 * b], row = ba.bad_list[2 * b + 1];
+        const int sp = ba.bad_list[3 * b], row = ba.bad_list[3 * b + 1];
+        const int mask = __builtin_amdgcn_readfirstlane(ba.bad_list[3 * b + 2]);
         const PairDev pd = ba.pairs[sp];
         const float *A = ba.cost + pd.a_off;
         const double *h = ba.other + pd.other_off;
         double *pot = ba.pot + pd.pot_off;
-        row_exact<E, THREADS>(A, pd.lda, row, pd.ncols, h, pd.other_ld, prm, red_a, red_b, l2_sh);
+        row_exact<E, THREADS>(A, pd.lda, row, pd.ncols, h, pd.other_ld, prm, mask, red_a, red_b, l2_sh);
+        // recomputed combinations take their new potential; the others keep the
+        // one the sweep wrote (a group's reference is recomputed with its group)
         if (tid < E) {
-            const double p = new_potential(ba, pd, prm[tid], row, sp, l2_sh[tid]);
+            const long long at = (long long)tid * pd.pot_ld + row;
+            const bool redo = (mask >> tid) & 1;
+            const double p = redo ? new_potential(ba, pd, prm[tid], row, sp, l2_sh[tid]) : pot[at];
             pot_sh[tid] = p;
-            pot[(long long)tid * pd.pot_ld + row] = p;
+            if (redo) pot[at] = p;
         }
         __syncthreads();
-        if (ba.group && tid < E) {
+        if (ba.group && tid < E && ((mask >> tid) & 1)) {
             const int ref = (tid / ba.group) * ba.group + ba.group / 2;
             float dpos;
             ba.fac[pd.pot_off + (long long)tid * pd.pot_ld + row] = factor_of(nxt[ref].k, pot_sh[tid], pot_sh[ref], &dpos);
@@ -792,7 +811,7 @@ struct BatchWork {
     int *n_src;
     int *iter, *queues;
     unsigned long long *exact_rows;
-    int *bad_f, *bad_g;      // repair lists: (side_pair, row) pairs
+    int *bad_f, *bad_g;      // repair lists: (side_pair, row, combination mask)
     float *fmax_f, *fmax_g;  // factor maxima per (pair, combination)
 };
 
@@ -823,8 +842,8 @@ static size_t batch_carve(Arena &ar, const wotk_batch_pair *pairs, int n_pairs, 
     w->pairs_g = ar.take<PairDev>(n_pairs);
     w->tiles_f = ar.take<Tile>(tf);
     w->tiles_g = ar.take<Tile>(tg);
-    w->bad_f = ar.take<int>(2 * tf);
-    w->bad_g = ar.take<int>(2 * tg);
+    w->bad_f = ar.take<int>(3 * tf);
+    w->bad_g = ar.take<int>(3 * tg);
     w->fmax_f = ar.take<float>((size_t)n_pairs * E);
     w->fmax_g = ar.take<float>((size_t)n_pairs * E);
     w->g_off = ar.take<long long>(n_pairs);
