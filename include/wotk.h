@@ -232,6 +232,50 @@ int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int64_t ldct, 
                     void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Populations through stored transport maps (TransportMapModel,
+ * wot/tmap/transport_map_model.py):
+ *   WOTK_PUSH_FORWARD  p <- p T        push_forward, :284-293
+ *   WOTK_PULL_BACK     p <- (T p^T)^T  pull_back,    :350-359
+ * maps[0..n_maps-1] are applied in that order (a pull back from day i to
+ * day i-2 passes T_{i-1,i} first, then T_{i-2,i-1}); map s is row-major
+ * (rows = source-day cells, cols = target-day cells), FP64 or FP32 entries,
+ * any leading dimension.  Consecutive maps must chain (push: cols of map s =
+ * rows of map s+1; pull: rows of map s = cols of map s+1).
+ * Populations are a CELLS x POPULATIONS matrix (ld ldp >= n_pops), i.e. the
+ * transpose of the reference's np.vstack([pop.p ...]).  With normalize != 0
+ * every population is divided by its sum after every map (the reference's
+ * normalize=True); a population summing to 0 becomes NaN there, as in numpy.
+ * outs is a HOST array of n_maps device pointers: outs[s] (may be NULL,
+ * except the last) receives the populations after map s as an n_s x n_pops
+ * matrix with leading dimension ldo.  Products and sums are FP64 (matrix
+ * cores); the split-reduction order is fixed, so results are deterministic.
+ * ------------------------------------------------------------------------- */
+#define WOTK_DTYPE_F64 0
+#define WOTK_DTYPE_F32 1
+#define WOTK_PUSH_FORWARD 0
+#define WOTK_PULL_BACK 1
+
+typedef struct wotk_tmap {
+    const void *data; /* device, row-major rows x cols, leading dimension ld */
+    int32_t dtype;    /* WOTK_DTYPE_F64 or WOTK_DTYPE_F32                    */
+    int32_t reserved; /* 0                                                   */
+    int64_t ld, rows, cols;
+} wotk_tmap;
+
+/* 0 when the arguments are invalid (wotk_tmap_propagate then reports why) */
+size_t wotk_tmap_workspace_size(const wotk_tmap *maps, int32_t n_maps, int32_t direction,
+                                int32_t n_pops);
+int wotk_tmap_propagate(const wotk_tmap *maps, int32_t n_maps, int32_t direction,
+                        const double *pops, int64_t ldp, int32_t n_pops, int32_t normalize,
+                        double *const *outs, int64_t ldo, void *workspace, size_t workspace_bytes,
+                        void *stream);
+
+/* X[i, 0:k] /= sum(X[i, 0:k]) for i < n (the fates matrix normalisation,
+ * transport_map_model.py:66 X /= X.sum(axis=1, keepdims=1)); a row summing
+ * to 0 becomes NaN, as in numpy. */
+int wotk_rows_normalize_f64(double *X, int64_t n, int32_t k, int64_t ld, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Tile shapes of the sweep kernels (no counterpart in the reference, which has
  * no device kernels).  A tile is (rows of the cost matrix per workgroup,
  * threads per workgroup).  The library keeps one tile per (kernel, shape

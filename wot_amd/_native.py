@@ -25,12 +25,23 @@ KERNEL_BATCH_SWEEP = 1
 KERNEL_BATCH_SHARED = 2
 BATCH_SHARE_EXP = 1
 
+DTYPE_F64 = 0
+DTYPE_F32 = 1
+PUSH_FORWARD = 0
+PULL_BACK = 1
+
 
 class Tile(ctypes.Structure):
     _fields_ = [('rows', ctypes.c_int32), ('threads', ctypes.c_int32)]
 
     def as_tuple(self):
         return (int(self.rows), int(self.threads))
+
+
+class TMap(ctypes.Structure):
+    """wotk_tmap: one stored transport map (device pointer, row-major)."""
+    _fields_ = [('data', ctypes.c_void_p), ('dtype', ctypes.c_int32), ('reserved', ctypes.c_int32),
+                ('ld', ctypes.c_int64), ('rows', ctypes.c_int64), ('cols', ctypes.c_int64)]
 
 
 class Stats(ctypes.Structure):
@@ -88,6 +99,10 @@ SIGNATURES = {
                                        _P, _P, _P, _P, _P, _SZ, _P]),
     'wotk_primal_dual': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _P, _P, _D, _D, _D,
                                         ctypes.POINTER(ctypes.c_double), _P, _SZ, _P]),
+    'wotk_tmap_workspace_size': (_SZ, [ctypes.POINTER(TMap), _I32, _I32, _I32]),
+    'wotk_tmap_propagate': (ctypes.c_int, [ctypes.POINTER(TMap), _I32, _I32, _P, _I64, _I32, _I32,
+                                           ctypes.POINTER(_P), _I64, _P, _SZ, _P]),
+    'wotk_rows_normalize_f64': (ctypes.c_int, [_P, _I64, _I32, _I64, _P]),
     'wotk_tile_variants': (_I32, [_I32, ctypes.POINTER(Tile), _I32]),
     'wotk_shape_bucket': (_I32, [_I32, _I64, _I64, _I32]),
     'wotk_num_buckets': (_I32, [_I32]),

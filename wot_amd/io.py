@@ -39,3 +39,21 @@ def read_npz_tmap(path):
     obs = pd.DataFrame({k[4:]: z[k] for k in z.files if k.startswith('obs_') and k != 'obs_index'},
                        index=pd.Index(z['obs_index']))
     return z['X'], obs, pd.Index(z['var_index'])
+
+
+def read_dataset(path):
+    """Read a transport map written by write_dataset ('txt' / 'csv' with an
+    'id' index column, or 'npz') as AnnData (the map formats of
+    wot/io/io.py:read_dataset that need no optional package)."""
+    from . import _anndata
+    p = str(path)
+    low = p.lower()
+    if low.endswith('.npz'):
+        X, obs, var = read_npz_tmap(p)
+        return _anndata.AnnData(np.asarray(X, dtype=np.float64), obs, pd.DataFrame(index=var))
+    if low.endswith('.txt') or low.endswith('.csv') or low.endswith('.tsv'):
+        sep = ',' if low.endswith('.csv') else '\t'
+        df = pd.read_csv(p, sep=sep, index_col=0)
+        return _anndata.AnnData(df.values.astype(np.float64), pd.DataFrame(index=df.index.astype(str)),
+                                pd.DataFrame(index=df.columns.astype(str)))
+    raise ValueError('Unknown file format for %s (supported: .npz, .txt, .tsv, .csv)' % p)

@@ -1,8 +1,14 @@
-"""Long-range couplings on the device (wot.tmap.glue_transport_maps /
-chain_transport_maps, wot/tmap/util.py:74-94 and wot/tmap/chaining.py:4-40).
+"""Transport maps on the device (wot.tmap).
 
-Maps are multiplied in FP32 on the matrix cores (``wotk_gemm_f32``) and kept
-in HBM between products; as in the reference there is no normalisation.
+* ``TransportMapModel`` (wot/tmap/transport_map_model.py): push_forward /
+  pull_back / fates / trajectories / transition_table / census over maps kept
+  in HBM, FP64 products on the matrix cores (``wotk_tmap_propagate``).
+* ``Population``, ``unique_timepoint``, ``find_path`` (wot/population.py,
+  wot/tmap/util.py:52-72, wot/tmap/chaining.py:43-110).
+* Long-range couplings (glue_transport_maps / chain_transport_maps,
+  wot/tmap/util.py:74-94 and wot/tmap/chaining.py:4-40): maps multiplied in
+  FP32 on the matrix cores (``wotk_gemm_f32``) and kept in HBM between
+  products; as in the reference there is no normalisation.
 """
 import ctypes
 
@@ -10,6 +16,9 @@ import numpy as np
 
 from .. import _native, _anndata
 from ..ot import device as _dev
+from .population import Population, find_path, unique_timepoint  # noqa: F401
+from .propagate import PULL_BACK, PUSH_FORWARD, DeviceMap, normalize_rows, propagate  # noqa: F401
+from .model import TransportMapModel  # noqa: F401
 
 
 def gemm_f32(A, B, out=None):
