@@ -33,9 +33,9 @@ SOLVER = dict(epsilon=0.05, lambda1=1.0, lambda2=50.0, epsilon0=1.0, tau=1e4, sc
               inner_iter_max=50, extra_iter=1000, growth_iters=3)
 
 
-def _traffic():
-    """HBM bytes per sweep launch from the calibrated PMC runs kept in profiles/."""
-    path = os.path.join(ROOT, 'profiles', 'lse_rows_kernel_15000x15000.traffic.json')
+def _traffic(name='lse_rows_kernel_15000x15000'):
+    """HBM bytes per launch of a kernel from the calibrated PMC runs kept in profiles/."""
+    path = os.path.join(ROOT, 'profiles', name + '.traffic.json')
     try:
         with open(path) as fh:
             return json.load(fh)['traffic_bytes']
@@ -549,6 +549,138 @@ def bench_large(args):
     return out
 
 
+TMAP_POPS = 10
+
+
+def bench_tmap(args):
+    """Trajectories (TransportMapModel.trajectories, transport_map_model.py:105-142)
+    of 10 populations at the middle day of the 39-day time course (cells per
+    day ~ U[1000, 20000], the batched time course's days): pull back through
+    19 maps and push forward through 19, every map read once per walk, every
+    day's populations written to one (cells x 10) matrix in HBM.  Maps are
+    synthetic FP64 (or --tmap-dtype float32) on the device.
+    value = trajectory walks per second; roofline per product kernel from the
+    library's hipEvent timer: algorithmic bytes = map bytes + 8 k (I + J)."""
+    import torch
+    import wot_amd.tmap as tm
+    from wot_amd.synthetic import time_course
+    _, _, counts = time_course(**TC)
+    counts = [int(c) for c in counts]
+    n_days = len(counts)
+    mid = n_days // 2
+    tdt = torch.float64 if args.tmap_dtype == 'float64' else torch.float32
+    esize = 8 if tdt == torch.float64 else 4
+    gen = torch.Generator(device='cuda').manual_seed(7)
+    maps = []
+    for t in range(n_days - 1):
+        m = tm.DeviceMap.empty(counts[t], counts[t + 1], args.tmap_dtype)
+        m.X.uniform_(generator=gen)
+        maps.append(m)
+    k = TMAP_POPS
+    P = (torch.rand((counts[mid], k), dtype=torch.float64, device='cuda', generator=gen) > 0.8).double()
+    P = P / P.sum(0, keepdim=True)
+    offs = np.cumsum([0] + counts)
+    X = torch.empty((int(offs[-1]), k), dtype=torch.float64, device='cuda')
+    torch.cuda.synchronize()
+    t_tr = time.perf_counter()
+    for t in range(mid):  # the pull walk streams transposed copies (TransportMapModel default)
+        maps[t].make_transposed()
+    torch.cuda.synchronize()
+    t_tr = time.perf_counter() - t_tr
+    back = [maps[t - 1] for t in range(mid, 0, -1)]
+    fwd = [maps[t] for t in range(mid, n_days - 1)]
+    out_back = [X[offs[t - 1]:offs[t]] for t in range(mid, 0, -1)]
+    out_fwd = [X[offs[t + 1]:offs[t + 2]] for t in range(mid, n_days - 1)]
+    map_bytes = {'pull': float(sum(esize * m.shape[0] * m.shape[1] for m in back)),
+                 'push': float(sum(esize * m.shape[0] * m.shape[1] for m in fwd))}
+    alg = {'pull': map_bytes['pull'] + 8.0 * k * sum(m.shape[0] + m.shape[1] for m in back),
+           'push': map_bytes['push'] + 8.0 * k * sum(m.shape[0] + m.shape[1] for m in fwd)}
+
+    def walk(stats=None):
+        sb, sf = ({}, {}) if stats is not None else (None, None)
+        X[offs[mid]:offs[mid + 1]].copy_(P)
+        tm.propagate(back, P, tm.PULL_BACK, normalize=True, outs=out_back, stats=sb)
+        tm.propagate(fwd, P, tm.PUSH_FORWARD, normalize=True, outs=out_fwd, stats=sf)
+        if stats is not None:
+            stats['pull'].append(sb['kernel_ms'])
+            stats['push'].append(sf['kernel_ms'])
+
+    for _ in range(max(args.warmup, 1)):
+        walk()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        walk()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    per = elapsed / args.steps
+    kt = {'pull': [], 'push': []}
+    for _ in range(args.steps):
+        walk(kt)
+    kern = {}
+    for d, name in (('pull', 'push_partial_kernel (transposed copies)'), ('push', 'push_partial_kernel')):
+        ms = float(np.mean(kt[d]))
+        n = len(back) if d == 'pull' else len(fwd)
+        kern[d] = {'kernel': name, 'launches': n, 'avg_launch_us': 1000.0 * ms / n,
+                   'achieved': alg[d] / (ms / 1000.0) / 1e9, 'alg_bytes_per_launch': alg[d] / n}
+    dom = min(kern, key=lambda d: kern[d]['achieved'])
+    roof = dict(bound='hbm', kernel=kern[dom]['kernel'], achieved=kern[dom]['achieved'], peak=HBM_PEAK_GBS,
+                unit='GB/s', frac=kern[dom]['achieved'] / HBM_PEAK_GBS,
+                traffic=_traffic('push_partial_kernel_timecourse_trajectories') if esize == 8 else None,
+                avg_launch_us=kern[dom]['avg_launch_us'])
+    total_bytes = alg['pull'] + alg['push']
+    out = {'metric': 'trajectory walks/sec (10 populations, middle day, 38 maps, every map streamed once)',
+           'value': 1.0 / per, 'unit': 'walks/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': max(args.warmup, 1),
+           'ms_per_step': 1000.0 * per, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+           'dtype': 'f64' if esize == 8 else 'f32 maps / f64 accumulate',
+           'data': 'synthetic (uniform random maps on the time course\'s day sizes)',
+           'config': {'workload': 'TransportMapModel.trajectories, 39 days, %d populations' % k,
+                      'cells_total': int(offs[-1]), 'map_bytes_total': map_bytes['pull'] + map_bytes['push'],
+                      'map_dtype': args.tmap_dtype, 'pull_maps': 'transposed copies in HBM',
+                      'transpose_once_ms': 1000.0 * t_tr},
+           'end_to_end_GBps': total_bytes / per / 1e9,
+           'roofline': roof, 'roofline_kernels': kern}
+    # parity on the run's own data: the adjoint identity <pull(q), p> = <q, push(p)> for one map
+    m = maps[mid]
+    pp = torch.rand((m.shape[0], 3), dtype=torch.float64, device='cuda', generator=gen)
+    qq = torch.rand((m.shape[1], 3), dtype=torch.float64, device='cuda', generator=gen)
+    push = tm.propagate([m], pp, tm.PUSH_FORWARD, normalize=False)
+    pull = tm.propagate([m], qq, tm.PULL_BACK, normalize=False)
+    lhs, rhs = (pull * pp).sum(0), (qq * push).sum(0)
+    out['adjoint_identity_rel'] = float(((lhs - rhs).abs() / rhs.abs()).max())
+    if not args.no_baselines:
+        out['cpu_baseline'] = _tmap_cpu_baseline(counts, k, total_bytes, per)
+    return out
+
+
+def _tmap_cpu_baseline(counts, k, total_bytes, gpu_per):
+    """The numpy oracle's push / pull (one dgemm per map, BLAS threads) on the
+    first maps of the same course, FP64, scaled to the full walk by bytes."""
+    from oracle import tmap_oracle as O
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max([i.get('num_threads', 1) for i in threadpool_info() if i.get('user_api') == 'blas'] or [1])
+    except Exception:
+        cores = os.cpu_count()
+    rng = np.random.default_rng(0)
+    maps = [rng.random((counts[t], counts[t + 1])) for t in range(3)]
+    P = rng.random((k, counts[0]))
+    Q = rng.random((k, counts[3]))
+    nbytes = float(sum(8 * m.size + 8 * k * (m.shape[0] + m.shape[1]) for m in maps))
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < 10.0:
+        O.push_forward(maps, P, 0, 3)
+        O.pull_back(maps, Q, 3, 0)
+        reps += 1
+    dt = (time.perf_counter() - t0) / reps
+    rate = 2 * nbytes / dt
+    return {'value': rate / total_bytes, 'unit': 'walks/s', 'cores': int(cores), 'kind': 'port',
+            'sample': 'push + pull of 10 populations through the first 3 maps (%s cells), %d reps, '
+                      'scaled by bytes to the 38-map walk' % ('-'.join(str(c) for c in counts[:4]), reps),
+            'GBps': rate / 1e9, 'gpu_over_cpu': (1.0 / gpu_per) / (rate / total_bytes)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -558,9 +690,12 @@ def main():
     ap.add_argument('--large-n', type=int, default=60000, help='large only: cells per day')
     ap.add_argument('--group-size', type=int, default=None,
                     help='timecourse only: combinations per batched run (default: the whole grid)')
-    ap.add_argument('--config', default='pair', choices=['pair', 'timecourse', 'adjoint', 'large'],
+    ap.add_argument('--config', default='pair', choices=['pair', 'timecourse', 'adjoint', 'large', 'tmap'],
                     help='pair: one 15k x 15k map (default); timecourse: 38 pairs x 9-point grid; '
-                         'adjoint: gradient of the converged objective, 8k x 8k')
+                         'adjoint: gradient of the converged objective, 8k x 8k; '
+                         'tmap: trajectories through the 38 maps of the time course')
+    ap.add_argument('--tmap-dtype', default='float64', choices=['float64', 'float32'],
+                    help='tmap only: precision of the stored maps')
     ap.add_argument('--cost-mode', default='bf16x6', choices=['f64', 'bf16x3', 'bf16x6'],
                     help='pair only: cost kernel of the timed maps')
     ap.add_argument('--no-tune', action='store_true',
@@ -580,6 +715,9 @@ def main():
         return
     if args.config == 'large':
         print(json.dumps(bench_large(args)), flush=True)
+        return
+    if args.config == 'tmap':
+        print(json.dumps(bench_tmap(args)), flush=True)
         return
 
     import torch

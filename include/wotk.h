@@ -249,6 +249,13 @@ int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int64_t ldct, 
  * except the last) receives the populations after map s as an n_s x n_pops
  * matrix with leading dimension ldo.  Products and sums are FP64 (matrix
  * cores); the split-reduction order is fixed, so results are deterministic.
+ * A pull back through a map with a transposed copy streams the copy with the
+ * push kernel (rows of the copy along the lanes, fully coalesced); without
+ * one it reads the map's rows directly.  Both give the same sums up to
+ * rounding order.
+ * stats (host, may be NULL): when given, every map's product kernel is timed
+ * with hipEvents and the call synchronises the stream before returning;
+ * sweeps = maps applied, sweep_ms = summed product-kernel time.
  * ------------------------------------------------------------------------- */
 #define WOTK_DTYPE_F64 0
 #define WOTK_DTYPE_F32 1
@@ -256,10 +263,12 @@ int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int64_t ldct, 
 #define WOTK_PULL_BACK 1
 
 typedef struct wotk_tmap {
-    const void *data; /* device, row-major rows x cols, leading dimension ld */
-    int32_t dtype;    /* WOTK_DTYPE_F64 or WOTK_DTYPE_F32                    */
-    int32_t reserved; /* 0                                                   */
+    const void *data;       /* device, row-major rows x cols, leading dimension ld      */
+    int32_t dtype;          /* WOTK_DTYPE_F64 or WOTK_DTYPE_F32                         */
+    int32_t reserved;       /* 0                                                        */
     int64_t ld, rows, cols;
+    const void *transposed; /* optional: the same map as cols x rows (same dtype), or NULL */
+    int64_t ld_t;           /* its leading dimension                                    */
 } wotk_tmap;
 
 /* 0 when the arguments are invalid (wotk_tmap_propagate then reports why) */
@@ -268,7 +277,12 @@ size_t wotk_tmap_workspace_size(const wotk_tmap *maps, int32_t n_maps, int32_t d
 int wotk_tmap_propagate(const wotk_tmap *maps, int32_t n_maps, int32_t direction,
                         const double *pops, int64_t ldp, int32_t n_pops, int32_t normalize,
                         double *const *outs, int64_t ldo, void *workspace, size_t workspace_bytes,
-                        void *stream);
+                        wotk_stats *stats, void *stream);
+
+/* dst (cols x rows, ld_dst) = src^T (src rows x cols, ld_src), FP64 or FP32
+ * (dtype): the transposed copy of a map for wotk_tmap.transposed. */
+int wotk_tmap_transpose(const void *src, int32_t dtype, int64_t ld_src, int64_t rows, int64_t cols,
+                        void *dst, int64_t ld_dst, void *stream);
 
 /* X[i, 0:k] /= sum(X[i, 0:k]) for i < n (the fates matrix normalisation,
  * transport_map_model.py:66 X /= X.sum(axis=1, keepdims=1)); a row summing

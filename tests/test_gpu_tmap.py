@@ -26,10 +26,13 @@ def _dev(x):
     return torch.from_numpy(np.ascontiguousarray(x)).cuda()
 
 
-def _walk(tm, maps_np, P, direction, normalize, dtype='float64', outs_all=False):
+def _walk(tm, maps_np, P, direction, normalize, dtype='float64', outs_all=False, transposed=False):
     """P: k x cells (reference layout).  Returns k x cells after the walk (and every step if outs_all)."""
     import torch
     dms = [tm.DeviceMap.upload(m, dtype) for m in maps_np]
+    if transposed:
+        for d in dms:
+            d.make_transposed()
     Pd = _dev(P.T.copy())
     k = P.shape[0]
     if outs_all:
@@ -89,9 +92,13 @@ def test_single_map_random(tm, dtype, k):
             got = _walk(tm, [M], P, tm.PUSH_FORWARD, normalize, dtype)
             e = rel(got, O.push_forward([Mref], P, 0, 1, normalize))
             assert e < 1e-13, ('push', a, b, k, normalize, e)
+            want = O.pull_back([Mref], Q, 1, 0, normalize)
             got = _walk(tm, [M], Q, tm.PULL_BACK, normalize, dtype)
-            e = rel(got, O.pull_back([Mref], Q, 1, 0, normalize))
+            e = rel(got, want)
             assert e < 1e-13, ('pull', a, b, k, normalize, e)
+            got = _walk(tm, [M], Q, tm.PULL_BACK, normalize, dtype, transposed=True)
+            e = rel(got, want)
+            assert e < 1e-13, ('pull via transposed copy', a, b, k, normalize, e)
 
 
 def test_chain_ragged_days_all_steps(tm):
@@ -103,10 +110,21 @@ def test_chain_ragged_days_all_steps(tm):
     for s, got in enumerate(steps):
         assert rel(got, O.push_forward(maps, P, 0, s + 1)) < 1e-13, s
     Q = rng.random((5, sizes[-1]))
-    steps = _walk(tm, maps[::-1], Q, tm.PULL_BACK, True, outs_all=True)
     last = len(sizes) - 1
-    for s, got in enumerate(steps):
-        assert rel(got, O.pull_back(maps, Q, last, last - s - 1)) < 1e-13, s
+    for transposed in (False, True):
+        steps = _walk(tm, maps[::-1], Q, tm.PULL_BACK, True, outs_all=True, transposed=transposed)
+        for s, got in enumerate(steps):
+            assert rel(got, O.pull_back(maps, Q, last, last - s - 1)) < 1e-13, (s, transposed)
+
+
+def test_transpose_is_exact(tm):
+    import torch
+    rng = np.random.default_rng(14)
+    for (a, b) in [(1, 1), (31, 33), (64, 64), (1000, 77), (129, 2050)]:
+        for dt in ('float64', 'float32'):
+            d = tm.DeviceMap.upload(rng.random((a, b)), dt).make_transposed()
+            assert d.XT.shape == (b, a)
+            assert torch.equal(d.XT, d.X.T)
 
 
 def test_strided_maps_take_the_scalar_path(tm):
@@ -166,6 +184,10 @@ def test_large_map_adjoint_and_mass(tm):
     lhs = (pull * P).sum(0)
     rhs = (Q * push).sum(0)
     assert torch.allclose(lhs, rhs, rtol=1e-12, atol=0)
+    dm.make_transposed()
+    pull_t = tm.propagate([dm], Q, tm.PULL_BACK, normalize=False)
+    assert torch.allclose(pull_t, pull, rtol=1e-12, atol=0)
+    dm.XT = None
     mass = (P * X.sum(1, keepdim=True)).sum(0)
     assert torch.allclose(push.sum(0), mass, rtol=1e-12, atol=0)
     # a sampled block against a library FP64 product
@@ -189,13 +211,13 @@ def _course(rng, sizes):
     return maps, meta, tmaps, ids
 
 
-@pytest.mark.parametrize('dtype', ['float64', 'float32'])
-def test_model_walks_match_oracle(tm, dtype):
+@pytest.mark.parametrize('dtype,transposed', [('float64', True), ('float32', True), ('float64', False)])
+def test_model_walks_match_oracle(tm, dtype, transposed):
     rng = np.random.default_rng(21)
     sizes = [120, 95, 200, 64, 150]
     maps, meta, tmaps, ids = _course(rng, sizes)
     mref = [m.astype(np.float32).astype(np.float64) for m in maps] if dtype == 'float32' else maps
-    model = tm.TransportMapModel(tmaps, meta, dtype=dtype)
+    model = tm.TransportMapModel(tmaps, meta, dtype=dtype, transposed_maps=transposed)
     pops = [tm.Population(1.0, rng.random(95), 'a'), tm.Population(1.0, (rng.random(95) > 0.7) * 1.0, 'b')]
     P = np.vstack([p.p for p in pops])
     out = model.push_forward(*pops, to_time=4.0)

@@ -41,7 +41,8 @@ class Tile(ctypes.Structure):
 class TMap(ctypes.Structure):
     """wotk_tmap: one stored transport map (device pointer, row-major)."""
     _fields_ = [('data', ctypes.c_void_p), ('dtype', ctypes.c_int32), ('reserved', ctypes.c_int32),
-                ('ld', ctypes.c_int64), ('rows', ctypes.c_int64), ('cols', ctypes.c_int64)]
+                ('ld', ctypes.c_int64), ('rows', ctypes.c_int64), ('cols', ctypes.c_int64),
+                ('transposed', ctypes.c_void_p), ('ld_t', ctypes.c_int64)]
 
 
 class Stats(ctypes.Structure):
@@ -101,8 +102,9 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_double), _P, _SZ, _P]),
     'wotk_tmap_workspace_size': (_SZ, [ctypes.POINTER(TMap), _I32, _I32, _I32]),
     'wotk_tmap_propagate': (ctypes.c_int, [ctypes.POINTER(TMap), _I32, _I32, _P, _I64, _I32, _I32,
-                                           ctypes.POINTER(_P), _I64, _P, _SZ, _P]),
+                                           ctypes.POINTER(_P), _I64, _P, _SZ, ctypes.POINTER(Stats), _P]),
     'wotk_rows_normalize_f64': (ctypes.c_int, [_P, _I64, _I32, _I64, _P]),
+    'wotk_tmap_transpose': (ctypes.c_int, [_P, _I32, _I64, _I64, _I64, _P, _I64, _P]),
     'wotk_tile_variants': (_I32, [_I32, ctypes.POINTER(Tile), _I32]),
     'wotk_shape_bucket': (_I32, [_I32, _I64, _I64, _I32]),
     'wotk_num_buckets': (_I32, [_I32]),

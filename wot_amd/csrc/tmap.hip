@@ -14,11 +14,12 @@
 //   FP32 16x16x4 form, whose rows are 4*(l/16) + r)
 // Push: A = a 4-row strip of T read with the columns on the lanes (coalesced
 // rows), B = the 4 matching population rows.  Pull: A = 16 rows of T, every
-// lane streaming its own 64 contiguous bytes of its row; the reduction index
+// lane streaming its own 32 contiguous bytes of its row; the reduction index
 // k is assigned to lanes so that each lane's bytes are contiguous, and B reads
 // the population rows of the same columns.  Long reductions are split over
 // workgroups into FP64 partials summed in a fixed order (deterministic).
 #include <algorithm>
+#include <vector>
 
 #include "common.h"
 
@@ -27,9 +28,13 @@ namespace {
 
 constexpr int kPopTile = 16;          // populations per MFMA tile
 constexpr int kWaves = 4;             // waves per workgroup (no LDS, no barriers)
-constexpr int kTargetWaves = 8192;    // ~8 waves per SIMD on 256 CUs
+constexpr int kTargetWaves = 4096;    // ~4 waves per SIMD on 256 CUs
+constexpr int kPushSteps = 4;         // MFMA k-steps (4 rows each) per push trip
+constexpr bool kPushPipelined = false; // push trips double-buffered (measured slower: fewer waves fit)
 constexpr int kPullRowBlocks = 4;     // 16-row MFMA blocks per pull wave
-constexpr int kReduceCells = 256;     // cells per reduce workgroup
+constexpr int kPullVecs = 2;          // 16-byte loads per lane, row and pull step
+constexpr bool kPullPipelined = true; // pull steps double-buffered
+constexpr int kReduceCells = 256 / 16; // cells per reduce workgroup (one thread per cell and population)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));
@@ -39,19 +44,27 @@ __device__ inline d4 mfma_f64(double a, double b, d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// B operand: population value, divided by the previous map's total when the
-// chain normalises (the reference divides p by its sum before the next map)
-__device__ inline double pop_value(const double *__restrict__ P, int64_t idx, const double *__restrict__ scale,
-                                   int pop) {
-    double v = P[idx];
-    return scale ? v / scale[pop] : v;
+// B operand scale: the reference divides every population by its sum before
+// the next map (p / sum); the products use p * (1 / sum), hoisted per lane
+__device__ inline double pop_scale(const double *__restrict__ scale, int group, int pop) {
+    return scale ? 1.0 / scale[group * kPopTile + pop] : 1.0;
+}
+
+// Map entries are read once per walk: non-temporal loads keep them from
+// evicting the population vectors (re-read by every workgroup) from L2.
+template <typename V>
+__device__ inline V stream_load(const V *p) {
+    return __builtin_nontemporal_load(p);
 }
 
 // ---------------------------------------------------------------------------
 // push: part[split][group][j][16] = sum_{i in split} T[i][j] * p[group][i][16]
 // One wave owns 64 columns of T.  FP64: the lane reads two double2 (columns
-// j0+2m.. and j0+32+2m..); FP32: one float4 (columns j0+4m..).  colmap gives
-// the column of MFMA block b at A-row m.
+// j0+2m.. and j0+32+2m..); FP32: one float4 (columns j0+4m..).  push_col
+// gives the column of MFMA block b at A-row m.  16 rows (4 MFMA k-steps) per
+// trip; the next trip's loads are issued before this trip's MFMAs.  FULL:
+// the wave's 64 columns are inside the map and rows are 16-byte aligned
+// (vector loads, no checks); otherwise element loads with column checks.
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ inline int64_t push_col(int64_t j0, int b, int m) {
@@ -59,25 +72,106 @@ __device__ inline int64_t push_col(int64_t j0, int b, int m) {
     else return j0 + 4 * m + b;
 }
 
-template <typename T, bool VEC>
-__device__ inline void push_load(const T *__restrict__ row, int64_t j0, int m, bool full, int64_t n_cols,
-                                 double a[4]) {
-    if (VEC && full) {
+// one lane's 4 entries of one map row, as loaded
+template <typename T> struct PushRow;
+template <> struct PushRow<double> {
+    d2 x, y;
+    __device__ double get(int b) const { return b < 2 ? x[b] : y[b - 2]; }
+};
+template <> struct PushRow<float> {
+    f4 x;
+    __device__ double get(int b) const { return (double)x[b]; }
+};
+
+template <typename T, bool FULL>
+__device__ inline void push_fetch(PushRow<T> &o, const T *__restrict__ row, int64_t j0, int m, int64_t n_cols) {
+    if constexpr (FULL) {
         if constexpr (sizeof(T) == 8) {
-            d2 x = *reinterpret_cast<const d2 *>(row + j0 + 2 * m);
-            d2 y = *reinterpret_cast<const d2 *>(row + j0 + 32 + 2 * m);
-            a[0] = x[0]; a[1] = x[1]; a[2] = y[0]; a[3] = y[1];
+            o.x = stream_load(reinterpret_cast<const d2 *>(row + j0 + 2 * m));
+            o.y = stream_load(reinterpret_cast<const d2 *>(row + j0 + 32 + 2 * m));
         } else {
-            f4 x = *reinterpret_cast<const f4 *>(row + j0 + 4 * m);
-            a[0] = x[0]; a[1] = x[1]; a[2] = x[2]; a[3] = x[3];
+            o.x = stream_load(reinterpret_cast<const f4 *>(row + j0 + 4 * m));
         }
     } else {
+        T v[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            int64_t j = push_col<T>(j0, b, m);
-            double v = (double)row[j < n_cols ? j : 0];
-            a[b] = j < n_cols ? v : 0.0;
+            const int64_t j = push_col<T>(j0, b, m);
+            const T e = row[j < n_cols ? j : 0];
+            v[b] = j < n_cols ? e : T(0);
         }
+        if constexpr (sizeof(T) == 8) {
+            o.x = d2{v[0], v[1]};
+            o.y = d2{v[2], v[3]};
+        } else {
+            o.x = f4{v[0], v[1], v[2], v[3]};
+        }
+    }
+}
+
+template <typename T>
+struct PushTrip {
+    PushRow<T> a[kPushSteps];
+    double b[kPushSteps];
+};
+
+template <typename T, bool FULL>
+__device__ inline void push_trip_fetch(PushTrip<T> &t, const T *__restrict__ map, int64_t ld, int64_t i,
+                                       const double *__restrict__ P, int64_t j0, int m, int q, int64_t n_cols) {
+#pragma unroll
+    for (int u = 0; u < kPushSteps; ++u) {
+        const int64_t r = i + 4 * u + q;
+        push_fetch<T, FULL>(t.a[u], map + r * ld, j0, m, n_cols);
+        t.b[u] = P[r * kPopTile + m];
+    }
+}
+
+template <typename T>
+__device__ inline void push_trip_mfma(const PushTrip<T> &t, double inv, d4 acc[4]) {
+#pragma unroll
+    for (int u = 0; u < kPushSteps; ++u) {
+        const double bv = t.b[u] * inv;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[b] = mfma_f64(t.a[u].get(b), bv, acc[b]);
+    }
+}
+
+template <typename T, bool FULL>
+__device__ inline void push_tile(const T *__restrict__ map, int64_t ld, int64_t i_begin, int64_t i_end,
+                                 int64_t n_cols, const double *__restrict__ P, double inv, int64_t j0, int m, int q,
+                                 d4 acc[4]) {
+    // full trips of 4 kPushSteps rows; pipelined: trip t+1 loads while trip t multiplies
+    constexpr int TR = 4 * kPushSteps;
+    const int64_t n_trips = (i_end - i_begin) / TR;
+    PushTrip<T> A;
+    if constexpr (kPushPipelined) {
+        PushTrip<T> B;
+        if (n_trips > 0) push_trip_fetch<T, FULL>(A, map, ld, i_begin, P, j0, m, q, n_cols);
+        int64_t t = 0;
+        for (; t + 2 <= n_trips; t += 2) {
+            push_trip_fetch<T, FULL>(B, map, ld, i_begin + (t + 1) * TR, P, j0, m, q, n_cols);
+            push_trip_mfma(A, inv, acc);
+            if (t + 2 < n_trips) push_trip_fetch<T, FULL>(A, map, ld, i_begin + (t + 2) * TR, P, j0, m, q, n_cols);
+            push_trip_mfma(B, inv, acc);
+        }
+        if (t < n_trips) push_trip_mfma(A, inv, acc);
+    } else {
+        for (int64_t t = 0; t < n_trips; ++t) {
+            push_trip_fetch<T, FULL>(A, map, ld, i_begin + t * TR, P, j0, m, q, n_cols);
+            push_trip_mfma(A, inv, acc);
+        }
+    }
+
+    // remaining rows, 4 per MFMA k-step
+    for (int64_t i = i_begin + n_trips * TR; i < i_end; i += 4) {
+        const int64_t r = i + q;
+        const bool ok = r < i_end;
+        const int64_t rr = ok ? r : i_begin;
+        PushRow<T> a;
+        push_fetch<T, FULL>(a, map + rr * ld, j0, m, n_cols);
+        const double bv = ok ? P[rr * kPopTile + m] * inv : 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[b] = mfma_f64(ok ? a.get(b) : 0.0, bv, acc[b]);
     }
 }
 
@@ -95,39 +189,16 @@ __global__ __launch_bounds__(256) void push_partial_kernel(const T *__restrict__
     const int64_t j0 = (int64_t)tile * 64;
     const int64_t i_begin = (int64_t)split * split_rows;
     const int64_t i_end = min(n_rows, i_begin + split_rows);
-    const bool full = j0 + 64 <= n_cols;
     const double *P = pops + (int64_t)group * n_rows * kPopTile;
-    const double *sc = scale ? scale + group * kPopTile : nullptr;
+    const double inv = pop_scale(scale, group, m);
 
     d4 acc[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[b] = d4{0.0, 0.0, 0.0, 0.0};
-
-    int64_t i = i_begin;
-    // main loop: 16 rows (4 MFMA k-steps) per trip, loads issued before the MFMAs
-    for (; i + 16 <= i_end; i += 16) {
-        double a[4][4], bv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int64_t r = i + 4 * u + q;
-            push_load<T, VEC>(map + r * ld, j0, m, full, n_cols, a[u]);
-            bv[u] = pop_value(P, r * kPopTile + m, sc, m);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[b] = mfma_f64(a[u][b], bv[u], acc[b]);
-    }
-    for (; i < i_end; i += 4) {
-        const int64_t r = i + q;
-        const bool ok = r < i_end;
-        const int64_t rr = ok ? r : i_begin;
-        double a[4];
-        push_load<T, VEC>(map + rr * ld, j0, m, full, n_cols, a);
-        const double bv = ok ? pop_value(P, rr * kPopTile + m, sc, m) : 0.0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[b] = mfma_f64(ok ? a[b] : 0.0, bv, acc[b]);
-    }
+    if (VEC && j0 + 64 <= n_cols)
+        push_tile<T, VEC>(map, ld, i_begin, i_end, n_cols, P, inv, j0, m, q, acc);
+    else
+        push_tile<T, false>(map, ld, i_begin, i_end, n_cols, P, inv, j0, m, q, acc);
 
     double *out = part + ((int64_t)split * n_groups + group) * n_cols * kPopTile;
 #pragma unroll
@@ -142,36 +213,87 @@ __global__ __launch_bounds__(256) void push_partial_kernel(const T *__restrict__
 // ---------------------------------------------------------------------------
 // pull: part[split][group][i][16] = sum_{j in split} T[i][j] * p[group][j][16]
 // One wave owns kPullRowBlocks x 16 rows.  Per step the lane reads E
-// contiguous entries (64 bytes) of its row at columns j + E*q .. + E-1; MFMA
-// s uses entry s, i.e. reduction index k = q <-> column j + E*q + s, and B
-// holds population row j + E*q + s.
+// contiguous entries (kPullVecs x 16 bytes) of its row at columns
+// j + E*q .. + E-1, so the 4 lanes of a row cover 4 E contiguous entries; MFMA s uses entry s, i.e.
+// reduction index k = q <-> column j + E*q + s, and B holds population row
+// j + E*q + s.  Full steps are double-buffered like the push trips; a partial
+// last step (element loads with column checks) follows them.
 // ---------------------------------------------------------------------------
-template <typename T, bool VEC>
-__device__ inline void pull_load(const T *__restrict__ row, int64_t jc, bool full, int64_t j_end,
-                                 double *a /* E */) {
-    constexpr int E = 64 / sizeof(T);
-    if (VEC && full) {
-        if constexpr (sizeof(T) == 8) {
+template <typename T> struct Vec16;  // 16 bytes of map entries
+template <> struct Vec16<double> { typedef d2 type; };
+template <> struct Vec16<float> { typedef f4 type; };
+
+// one lane's E = kPullVecs * 16 / sizeof(T) contiguous entries of one row
+template <typename T>
+struct PullRow {
+    static constexpr int PER = 16 / sizeof(T);  // entries per vector
+    static constexpr int E = kPullVecs * PER;
+    typename Vec16<T>::type v[kPullVecs];
+    __device__ double get(int s) const { return (double)v[s / PER][s % PER]; }
+};
+
+// VEC: 16-byte vector loads; CHECK: columns may reach past j_end
+template <typename T, bool VEC, bool CHECK>
+__device__ inline void pull_fetch(PullRow<T> &o, const T *__restrict__ row, int64_t jc, int64_t j_end) {
+    typedef typename Vec16<T>::type V;
+    constexpr int PER = PullRow<T>::PER;
+    if constexpr (VEC && !CHECK) {
 #pragma unroll
-            for (int v = 0; v < E / 2; ++v) {
-                d2 x = *reinterpret_cast<const d2 *>(row + jc + 2 * v);
-                a[2 * v] = x[0];
-                a[2 * v + 1] = x[1];
-            }
-        } else {
-#pragma unroll
-            for (int v = 0; v < E / 4; ++v) {
-                f4 x = *reinterpret_cast<const f4 *>(row + jc + 4 * v);
-                a[4 * v] = x[0]; a[4 * v + 1] = x[1]; a[4 * v + 2] = x[2]; a[4 * v + 3] = x[3];
-            }
-        }
+        for (int v = 0; v < kPullVecs; ++v) o.v[v] = stream_load(reinterpret_cast<const V *>(row + jc + v * PER));
     } else {
 #pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const int64_t j = jc + s;
-            const double v = (double)row[j < j_end ? j : 0];
-            a[s] = j < j_end ? v : 0.0;
+        for (int v = 0; v < kPullVecs; ++v)
+#pragma unroll
+            for (int e = 0; e < PER; ++e) {
+                const int64_t j = jc + v * PER + e;
+                if constexpr (CHECK) {
+                    const T x = row[j < j_end ? j : 0];
+                    o.v[v][e] = j < j_end ? x : T(0);
+                } else {
+                    o.v[v][e] = row[j];
+                }
+            }
+    }
+}
+
+template <typename T>
+struct PullStep {
+    static constexpr int E = PullRow<T>::E;
+    PullRow<T> a[kPullRowBlocks];
+    double b[E];
+};
+
+template <typename T, bool VEC, bool CHECK>
+__device__ inline void pull_step_fetch(PullStep<T> &st, const T *const rows[kPullRowBlocks],
+                                       const bool row_ok[kPullRowBlocks], int64_t j, int64_t j_begin, int64_t j_end,
+                                       const double *__restrict__ P, int m, int q) {
+    constexpr int E = PullStep<T>::E;
+    const int64_t jc = j + (int64_t)E * q;
+#pragma unroll
+    for (int rb = 0; rb < kPullRowBlocks; ++rb) {
+        pull_fetch<T, VEC, CHECK>(st.a[rb], rows[rb], jc, j_end);
+        if (!row_ok[rb]) st.a[rb] = PullRow<T>{};
+    }
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int64_t c = jc + s;
+        if constexpr (CHECK) {
+            const bool ok = c < j_end;
+            const double v = P[(ok ? c : j_begin) * kPopTile + m];
+            st.b[s] = ok ? v : 0.0;
+        } else {
+            st.b[s] = P[c * kPopTile + m];
         }
+    }
+}
+
+template <typename T>
+__device__ inline void pull_step_mfma(const PullStep<T> &st, double inv, d4 acc[kPullRowBlocks]) {
+#pragma unroll
+    for (int s = 0; s < PullStep<T>::E; ++s) {
+        const double bv = st.b[s] * inv;
+#pragma unroll
+        for (int rb = 0; rb < kPullRowBlocks; ++rb) acc[rb] = mfma_f64(st.a[rb].get(s), bv, acc[rb]);
     }
 }
 
@@ -181,8 +303,7 @@ __global__ __launch_bounds__(256) void pull_partial_kernel(const T *__restrict__
                                                            const double *__restrict__ scale,
                                                            double *__restrict__ part, int64_t split_cols,
                                                            int n_tiles) {
-    constexpr int E = 64 / sizeof(T);  // entries per lane per step
-    constexpr int KS = 4 * E;          // columns per step
+    constexpr int KS = 4 * PullStep<T>::E;  // columns per step
     constexpr int RB = kPullRowBlocks;
     const int lane = threadIdx.x & 63;
     const int tile = blockIdx.x * kWaves + (threadIdx.x >> 6);
@@ -193,7 +314,7 @@ __global__ __launch_bounds__(256) void pull_partial_kernel(const T *__restrict__
     const int64_t j_begin = (int64_t)split * split_cols;
     const int64_t j_end = min(n_cols, j_begin + split_cols);
     const double *P = pops + (int64_t)group * n_cols * kPopTile;
-    const double *sc = scale ? scale + group * kPopTile : nullptr;
+    const double inv = pop_scale(scale, group, m);
 
     const T *rows[RB];
     bool row_ok[RB];
@@ -207,28 +328,29 @@ __global__ __launch_bounds__(256) void pull_partial_kernel(const T *__restrict__
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
 
-    for (int64_t j = j_begin; j < j_end; j += KS) {
-        const bool full = j + KS <= j_end;
-        const int64_t jc = j + (int64_t)E * q;
-        double a[RB][E], bv[E];
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-            pull_load<T, VEC>(rows[rb], jc, full, j_end, a[rb]);
-            if (!row_ok[rb])
-#pragma unroll
-                for (int s = 0; s < E; ++s) a[rb][s] = 0.0;
+    const int64_t n_full = (j_end - j_begin) / KS;
+    PullStep<T> A;
+    if constexpr (kPullPipelined) {
+        PullStep<T> B;
+        if (n_full > 0) pull_step_fetch<T, VEC, false>(A, rows, row_ok, j_begin, j_begin, j_end, P, m, q);
+        int64_t t = 0;
+        for (; t + 2 <= n_full; t += 2) {
+            pull_step_fetch<T, VEC, false>(B, rows, row_ok, j_begin + (t + 1) * KS, j_begin, j_end, P, m, q);
+            pull_step_mfma(A, inv, acc);
+            if (t + 2 < n_full)
+                pull_step_fetch<T, VEC, false>(A, rows, row_ok, j_begin + (t + 2) * KS, j_begin, j_end, P, m, q);
+            pull_step_mfma(B, inv, acc);
         }
-#pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const int64_t c = jc + s;
-            const bool ok = c < j_end;
-            const double v = pop_value(P, (ok ? c : j_begin) * kPopTile + m, sc, m);
-            bv[s] = ok ? v : 0.0;
+        if (t < n_full) pull_step_mfma(A, inv, acc);
+    } else {
+        for (int64_t t = 0; t < n_full; ++t) {
+            pull_step_fetch<T, VEC, false>(A, rows, row_ok, j_begin + t * KS, j_begin, j_end, P, m, q);
+            pull_step_mfma(A, inv, acc);
         }
-#pragma unroll
-        for (int s = 0; s < E; ++s)
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(a[rb][s], bv[s], acc[rb]);
+    }
+    if (j_begin + n_full * KS < j_end) {
+        pull_step_fetch<T, VEC, true>(A, rows, row_ok, j_begin + n_full * KS, j_begin, j_end, P, m, q);
+        pull_step_mfma(A, inv, acc);
     }
 
     double *out = part + ((int64_t)split * n_groups + group) * n_rows * kPopTile;
@@ -255,24 +377,29 @@ __global__ __launch_bounds__(256) void pops_import_kernel(const double *__restri
     dst[(int64_t)group * n * kPopTile + t] = pop < k ? src[cell * ld + pop] : 0.0;
 }
 
-// sum of the split partials in split order; per-workgroup partial sums of
-// every population for the normalisation (fixed order: deterministic)
+// sum of the split partials in split order, one thread per (cell, population);
+// per-workgroup partial sums of every population (16 cells) for the
+// normalisation (fixed order: deterministic)
 __global__ __launch_bounds__(256) void pops_reduce_kernel(const double *__restrict__ part, int n_split, int64_t n,
                                                           double *__restrict__ dst, double *__restrict__ block_sums) {
     __shared__ double red[256];
     const int group = blockIdx.y, n_groups = gridDim.y;
-    const int pop = threadIdx.x % kPopTile;
-    const int64_t cell0 = (int64_t)blockIdx.x * kReduceCells;
-    double tot = 0.0;
-    for (int c = threadIdx.x / kPopTile; c < kReduceCells; c += 256 / kPopTile) {
-        const int64_t cell = cell0 + c;
-        if (cell >= n) break;
-        double v = 0.0;
-        for (int s = 0; s < n_split; ++s) v += part[(((int64_t)s * n_groups + group) * n + cell) * kPopTile + pop];
-        dst[((int64_t)group * n + cell) * kPopTile + pop] = v;
-        tot += v;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;  // cell * 16 + pop
+    const int64_t stride = n * kPopTile;                        // one split's slice of a group
+    double v = 0.0;
+    if (t < stride) {
+        const double *src = part + (int64_t)group * stride + t;
+        const int64_t step = (int64_t)n_groups * stride;
+        int s = 0;
+        for (; s + 4 <= n_split; s += 4) {
+            const double a = src[(int64_t)s * step], b = src[(int64_t)(s + 1) * step];
+            const double c = src[(int64_t)(s + 2) * step], d = src[(int64_t)(s + 3) * step];
+            v = (((v + a) + b) + c) + d;
+        }
+        for (; s < n_split; ++s) v += src[(int64_t)s * step];
+        dst[(int64_t)group * stride + t] = v;
     }
-    red[threadIdx.x] = tot;
+    red[threadIdx.x] = v;
     __syncthreads();
     for (int w = 128; w >= kPopTile; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
@@ -282,18 +409,26 @@ __global__ __launch_bounds__(256) void pops_reduce_kernel(const double *__restri
         block_sums[((int64_t)group * gridDim.x + blockIdx.x) * kPopTile + threadIdx.x] = red[threadIdx.x];
 }
 
-// totals[group*16 + pop] = sum of the block sums (padded populations: 1)
-__global__ __launch_bounds__(256) void pops_totals_kernel(const double *__restrict__ block_sums, int n_blocks, int k,
-                                                          double *__restrict__ totals) {
-    __shared__ double red[256];
+// totals[group*16 + pop] = sum of the block sums (padded populations: 1);
+// 1024 threads, 4 independent partial sums each, combined in a fixed order
+constexpr int kTotalsThreads = 1024;
+__global__ __launch_bounds__(kTotalsThreads) void pops_totals_kernel(const double *__restrict__ block_sums,
+                                                                     int n_blocks, int k,
+                                                                     double *__restrict__ totals) {
+    __shared__ double red[kTotalsThreads];
     const int group = blockIdx.x;
     const int pop = threadIdx.x % kPopTile;
-    double t = 0.0;
-    for (int b = threadIdx.x / kPopTile; b < n_blocks; b += 256 / kPopTile)
-        t += block_sums[((int64_t)group * n_blocks + b) * kPopTile + pop];
-    red[threadIdx.x] = t;
+    constexpr int lanes = kTotalsThreads / kPopTile;  // threads per population
+    const double *src = block_sums + (int64_t)group * n_blocks * kPopTile + pop;
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    int b = threadIdx.x / kPopTile;
+    for (; b + 3 * lanes < n_blocks; b += 4 * lanes)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) t[u] += src[(int64_t)(b + u * lanes) * kPopTile];
+    for (; b < n_blocks; b += lanes) t[0] += src[(int64_t)b * kPopTile];
+    red[threadIdx.x] = (t[0] + t[1]) + (t[2] + t[3]);
     __syncthreads();
-    for (int w = 128; w >= kPopTile; w >>= 1) {
+    for (int w = kTotalsThreads / 2; w >= kPopTile; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
@@ -317,6 +452,26 @@ __global__ __launch_bounds__(256) void pops_export_kernel(const double *__restri
     dst[cell * ld + pop] = totals ? v / totals[pop] : v;
 }
 
+// dst[j][i] = src[i][j] through a padded 32 x 32 LDS tile (reads and writes coalesced)
+template <typename T>
+__global__ __launch_bounds__(256) void transpose_kernel(const T *__restrict__ src, int64_t lds, int64_t rows,
+                                                        int64_t cols, T *__restrict__ dst, int64_t ldd) {
+    __shared__ T tile[32][33];
+    const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int64_t r = r0 + ty + k, c = c0 + tx;
+        if (r < rows && c < cols) tile[ty + k][tx] = src[r * lds + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int64_t c = c0 + ty + k, r = r0 + tx;  // dst row c, column r
+        if (c < cols && r < rows) dst[c * ldd + r] = tile[tx][ty + k];
+    }
+}
+
 // X[i, :k] /= sum(X[i, :k])   (fates: X /= X.sum(axis=1), transport_map_model.py:66)
 __global__ __launch_bounds__(256) void rows_normalize_kernel(double *__restrict__ X, int64_t n, int k, int64_t ld) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -337,7 +492,23 @@ struct StepShape {
     int64_t split_len;
 };
 
-inline int pull_step_cols(int dtype) { return dtype == WOTK_DTYPE_F64 ? 32 : 64; }
+inline int pull_step_cols(int dtype) { return 4 * kPullVecs * (dtype == WOTK_DTYPE_F64 ? 2 : 4); }
+
+// a pull back through a map with a transposed copy is a push forward through the copy
+wotk_tmap effective_map(const wotk_tmap &m, int direction, int *eff_direction) {
+    if (direction == WOTK_PULL_BACK && m.transposed) {
+        wotk_tmap t = m;
+        t.data = m.transposed;
+        t.ld = m.ld_t;
+        t.rows = m.cols;
+        t.cols = m.rows;
+        t.transposed = nullptr;
+        *eff_direction = WOTK_PUSH_FORWARD;
+        return t;
+    }
+    *eff_direction = direction;
+    return m;
+}
 
 StepShape step_shape(const wotk_tmap &m, int direction, int n_groups) {
     StepShape s;
@@ -348,9 +519,9 @@ StepShape step_shape(const wotk_tmap &m, int direction, int n_groups) {
     const int64_t tiles = push ? (m.cols + 63) / 64 : (m.rows + 16 * kPullRowBlocks - 1) / (16 * kPullRowBlocks);
     s.n_tiles = (int)tiles;
     const int64_t quantum = push ? 16 : pull_step_cols(m.dtype);
-    // enough splits to put ~kTargetWaves waves in flight, each reduction chunk >= 4 quanta
+    // enough splits to put ~kTargetWaves waves in flight, each reduction chunk >= 8 quanta
     int64_t want = (kTargetWaves + tiles * n_groups - 1) / (tiles * n_groups);
-    int64_t most = std::max<int64_t>(1, s.reduce_len / (4 * quantum));
+    int64_t most = std::max<int64_t>(1, s.reduce_len / (8 * quantum));
     int64_t splits = std::max<int64_t>(1, std::min(want, most));
     s.split_len = round_up((s.reduce_len + splits - 1) / splits, quantum);
     if (s.split_len <= 0) s.split_len = quantum;
@@ -372,6 +543,8 @@ int check_chain(const wotk_tmap *maps, int32_t n_maps, int32_t direction, int32_
         WOTK_REQUIRE(m.data && m.rows > 0 && m.cols > 0 && m.ld >= m.cols, "tmap: map %d has a bad shape", s);
         WOTK_REQUIRE(m.dtype == WOTK_DTYPE_F64 || m.dtype == WOTK_DTYPE_F32, "tmap: map %d has dtype %d", s,
                      m.dtype);
+        WOTK_REQUIRE(!m.transposed || m.ld_t >= m.rows, "tmap: map %d has a transposed copy with ld %lld < %lld",
+                     s, (long long)m.ld_t, (long long)m.rows);
         if (s > 0) {
             const wotk_tmap &p = maps[s - 1];
             const int64_t prev_out = direction == WOTK_PUSH_FORWARD ? p.cols : p.rows;
@@ -387,7 +560,9 @@ Layout carve(Arena &ar, const wotk_tmap *maps, int32_t n_maps, int32_t direction
     const int n_groups = (n_pops + kPopTile - 1) / kPopTile;
     int64_t max_cells = 0, max_part = 0, max_blocks = 0;
     for (int s = 0; s < n_maps; ++s) {
-        StepShape sh = step_shape(maps[s], direction, n_groups);
+        int dir;
+        const wotk_tmap m = effective_map(maps[s], direction, &dir);
+        StepShape sh = step_shape(m, dir, n_groups);
         max_cells = std::max({max_cells, sh.n_in, sh.n_out});
         max_part = std::max(max_part, (int64_t)sh.n_split * sh.n_out);
         max_blocks = std::max(max_blocks, (sh.n_out + kReduceCells - 1) / kReduceCells);
@@ -444,7 +619,8 @@ extern "C" size_t wotk_tmap_workspace_size(const wotk_tmap *maps, int32_t n_maps
 
 extern "C" int wotk_tmap_propagate(const wotk_tmap *maps, int32_t n_maps, int32_t direction, const double *pops,
                                    int64_t ldp, int32_t n_pops, int32_t normalize, double *const *outs,
-                                   int64_t ldo, void *workspace, size_t workspace_bytes, void *stream) {
+                                   int64_t ldo, void *workspace, size_t workspace_bytes, wotk_stats *stats,
+                                   void *stream) {
     int rc = check_chain(maps, n_maps, direction, n_pops);
     if (rc != WOTK_OK) return rc;
     WOTK_REQUIRE(pops && ldp >= n_pops, "tmap: bad population matrix (ld %lld for %d populations)",
@@ -463,16 +639,25 @@ extern "C" int wotk_tmap_propagate(const wotk_tmap *maps, int32_t n_maps, int32_
         pops_import_kernel<<<grid, 256, 0, st>>>(pops, ldp, n0, n_pops, L.pop_a);
         WOTK_LAUNCH_CHECK();
     }
+    // optional timer: one event pair around every map's product kernel
+    std::vector<hipEvent_t> ev;
+    if (stats) {
+        ev.resize(2 * (size_t)n_maps);
+        for (auto &e : ev) WOTK_HIP(hipEventCreate(&e));
+    }
     double *cur = L.pop_a, *nxt = L.pop_b;
     const double *scale = nullptr;  // totals of the previous map (normalising chains)
     for (int s = 0; s < n_maps; ++s) {
-        const wotk_tmap &m = maps[s];
-        StepShape sh = step_shape(m, direction, n_groups);
+        int dir;
+        const wotk_tmap m = effective_map(maps[s], direction, &dir);
+        StepShape sh = step_shape(m, dir, n_groups);
+        if (stats) WOTK_HIP(hipEventRecord(ev[2 * s], st));
         if (m.dtype == WOTK_DTYPE_F64)
-            launch_partial<double>(m, direction, sh, n_groups, cur, scale, L.part, st);
+            launch_partial<double>(m, dir, sh, n_groups, cur, scale, L.part, st);
         else
-            launch_partial<float>(m, direction, sh, n_groups, cur, scale, L.part, st);
+            launch_partial<float>(m, dir, sh, n_groups, cur, scale, L.part, st);
         WOTK_LAUNCH_CHECK();
+        if (stats) WOTK_HIP(hipEventRecord(ev[2 * s + 1], st));
         const int n_blocks = (int)((sh.n_out + kReduceCells - 1) / kReduceCells);
         pops_reduce_kernel<<<dim3(n_blocks, n_groups), 256, 0, st>>>(L.part, sh.n_split, sh.n_out, nxt,
                                                                     L.block_sums);
@@ -480,7 +665,7 @@ extern "C" int wotk_tmap_propagate(const wotk_tmap *maps, int32_t n_maps, int32_
         double *tot = nullptr;
         if (normalize) {
             tot = L.totals + (s & 1) * n_groups * kPopTile;
-            pops_totals_kernel<<<n_groups, 256, 0, st>>>(L.block_sums, n_blocks, n_pops, tot);
+            pops_totals_kernel<<<n_groups, kTotalsThreads, 0, st>>>(L.block_sums, n_blocks, n_pops, tot);
             WOTK_LAUNCH_CHECK();
         }
         if (outs[s]) {
@@ -491,6 +676,38 @@ extern "C" int wotk_tmap_propagate(const wotk_tmap *maps, int32_t n_maps, int32_
         scale = tot;
         std::swap(cur, nxt);
     }
+    if (stats) {
+        *stats = wotk_stats{};
+        WOTK_HIP(hipStreamSynchronize(st));
+        double total = 0.0;
+        for (int s = 0; s < n_maps; ++s) {
+            float ms = 0.f;
+            WOTK_HIP(hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]));
+            total += ms;
+        }
+        for (auto &e : ev) WOTK_HIP(hipEventDestroy(e));
+        stats->iterations = n_maps;
+        stats->sweeps = n_maps;
+        stats->sweep_ms = total;
+    }
+    return WOTK_OK;
+}
+
+extern "C" int wotk_tmap_transpose(const void *src, int32_t dtype, int64_t ld_src, int64_t rows, int64_t cols,
+                                   void *dst, int64_t ld_dst, void *stream) {
+    WOTK_REQUIRE(src && dst && rows > 0 && cols > 0 && ld_src >= cols && ld_dst >= rows,
+                 "tmap_transpose: bad arguments");
+    WOTK_REQUIRE(dtype == WOTK_DTYPE_F64 || dtype == WOTK_DTYPE_F32, "tmap_transpose: bad dtype %d", dtype);
+    WOTK_REQUIRE((cols + 31) / 32 < (1ll << 31) && (rows + 31) / 32 < 65536, "tmap_transpose: map too large");
+    dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == WOTK_DTYPE_F64)
+        transpose_kernel<double><<<grid, 256, 0, st>>>(static_cast<const double *>(src), ld_src, rows, cols,
+                                                      static_cast<double *>(dst), ld_dst);
+    else
+        transpose_kernel<float><<<grid, 256, 0, st>>>(static_cast<const float *>(src), ld_src, rows, cols,
+                                                     static_cast<float *>(dst), ld_dst);
+    WOTK_LAUNCH_CHECK();
     return WOTK_OK;
 }
 

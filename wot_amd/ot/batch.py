@@ -35,16 +35,23 @@ class BatchResult:
         self.shapes = shapes
         self.eps_final = eps_final  # per combination
 
-    def plan(self, p, e, device_out=False):
-        """Transport map of pair p, combination e: exp((f+g-C)/eps) / J (FP64)."""
+    def plan(self, p, e, device_out=False, out=None):
+        """Transport map of pair p, combination e: exp((f+g-C)/eps) / J (FP64).
+        ``out``: an (I, J) float64 device view with unit column stride to write
+        into (any row stride); implies device_out."""
         torch = _dev.torch_mod()
         bp = self.pairs[p]
         n_i, n_j = self.shapes[p]
-        out = torch.empty((n_i, n_j), dtype=torch.float64, device=self.cost_buf.device)
+        if out is None:
+            out = torch.empty((n_i, n_j), dtype=torch.float64, device=self.cost_buf.device)
+        elif out.dtype != torch.float64 or tuple(out.shape) != (n_i, n_j) or out.stride(1) != 1:
+            raise ValueError('out must be a float64 (%d, %d) view with unit column stride' % (n_i, n_j))
+        else:
+            device_out = True
         C = self.cost_buf[bp.c_offset:]
         _native.check(_native.lib().wotk_plan(_dev._ptr(C), bp.ldc, n_i, n_j, _dev._ptr(self.f[p][e]),
                                               _dev._ptr(self.g[p][e]), float(self.eps_final[e]), float(n_j),
-                                              _dev._ptr(out), n_j, None, _dev._stream(torch)))
+                                              _dev._ptr(out), out.stride(0), None, _dev._stream(torch)))
         return out if device_out else out.cpu().numpy()
 
     def row_sums(self, p, e):

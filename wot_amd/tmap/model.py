@@ -28,10 +28,13 @@ class TransportMapModel:
         of the maps
     dtype : 'float64' (default, the reference's precision) or 'float32' for the
         device copies of the maps
+    transposed_maps : keep a transposed copy of every map in HBM (default), so
+        that pull backs stream rows as coalesced as push forwards; False halves
+        the memory and pulls read the maps' rows directly
     """
 
     def __init__(self, tmaps, meta, timepoints=None, day_pairs=None, cache=False, dtype='float64',
-                 device='cuda'):
+                 device='cuda', transposed_maps=True):
         self.tmaps = tmaps
         self.meta = meta
         self.cache = cache
@@ -43,30 +46,32 @@ class TransportMapModel:
         self.day_pairs = day_pairs
         self.dtype = dtype
         self.device = device
+        self.transposed_maps = transposed_maps
         self._device_maps = {}
 
     # ------------------------------------------------------------------
     # maps
     # ------------------------------------------------------------------
     @classmethod
-    def from_time_course(cls, result, days, meta, combo=0, dtype='float64'):
+    def from_time_course(cls, result, days, meta, combo=0, dtype='float64', transposed_maps=True):
         """Model over the maps of a batched time-course solve
         (``wot_amd.ot.batch.solve_time_course``): every pair's map of
         combination ``combo`` is materialised on the device from its
         potentials and never leaves HBM.  ``days[p]`` is the day of the p-th
         coordinate block; ``meta`` lists the cells day by day in that order."""
-        torch = _dev.torch_mod()
         tmaps = {}
-        tdt = torch.float64 if dtype == 'float64' else torch.float32
-        for p, bp in enumerate(result.pairs):
+        for p in range(len(result.pairs)):
             t0, t1 = days[p], days[p + 1]
-            X = result.plan(p, combo, device_out=True)
-            if tdt != torch.float64:
-                X = X.to(tdt)
-            ids0 = meta.index[meta['day'] == t0]
-            ids1 = meta.index[meta['day'] == t1]
-            tmaps[(t0, t1)] = DeviceMap(X, ids0, ids1)
-        return cls(tmaps, meta, timepoints=list(days), dtype=dtype)
+            n_i, n_j = result.shapes[p]
+            dm = DeviceMap.empty(n_i, n_j, 'float64', obs_names=meta.index[meta['day'] == t0],
+                                 var_names=meta.index[meta['day'] == t1])
+            result.plan(p, combo, out=dm.X)
+            if dtype == 'float32':
+                f32 = DeviceMap.empty(n_i, n_j, 'float32', obs_names=dm.obs_names, var_names=dm.var_names)
+                f32.X.copy_(dm.X)
+                dm = f32
+            tmaps[(t0, t1)] = dm
+        return cls(tmaps, meta, timepoints=list(days), dtype=dtype, transposed_maps=transposed_maps)
 
     def device_map(self, t0, t1):
         """The map (t0, t1) on the device, uploaded on first use and kept."""
@@ -79,9 +84,12 @@ class TransportMapModel:
             if isinstance(src, DeviceMap):
                 dm = src
                 if self.dtype == 'float32' and src.X.element_size() == 8:
-                    dm = DeviceMap(src.X.float(), src.obs_names, src.var_names)
+                    dm = DeviceMap.upload(src.X, 'float32', self.device)
+                    dm.obs_names, dm.var_names = src.obs_names, src.var_names
             else:
                 dm = DeviceMap.upload(self._as_anndata(src, key), self.dtype, self.device)
+            if self.transposed_maps:
+                dm.make_transposed()
             self._device_maps[key] = dm
         return dm
 

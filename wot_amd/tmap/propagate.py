@@ -34,6 +34,34 @@ class DeviceMap:
         self.X = X
         self.obs_names = obs_names
         self.var_names = var_names
+        self.XT = None  # optional transposed copy (cols x rows): pull backs stream it coalesced
+
+    def make_transposed(self):
+        """Keep a transposed copy in HBM (doubles the map's memory): every pull
+        back through this map then runs the push kernel over the copy."""
+        if self.XT is None:
+            torch = _dev.torch_mod()
+            dt = 'float64' if self.X.dtype == torch.float64 else 'float32'
+            t = DeviceMap.empty(self.X.shape[1], self.X.shape[0], dt, self.X.device).X
+            code = _native.DTYPE_F64 if dt == 'float64' else _native.DTYPE_F32
+            _native.check(_native.lib().wotk_tmap_transpose(_dev._ptr(self.X), code, self.X.stride(0),
+                                                            self.X.shape[0], self.X.shape[1], _dev._ptr(t),
+                                                            t.stride(0), _dev._stream(torch)))
+            self.XT = t
+        return self
+
+    @classmethod
+    def empty(cls, rows, cols, dtype='float64', device='cuda', obs_names=None, var_names=None):
+        """Uninitialised map whose rows start on 256-byte boundaries (the
+        product kernels then read every row with 16-byte vector loads)."""
+        torch = _dev.torch_mod()
+        if dtype not in _DTYPES:
+            raise ValueError('dtype must be float64 or float32, not %r' % (dtype,))
+        tdt = torch.float64 if dtype == 'float64' else torch.float32
+        align = 32 if dtype == 'float64' else 64
+        ld = (int(cols) + align - 1) // align * align
+        buf = torch.empty((int(rows), ld), dtype=tdt, device=device)
+        return cls(buf[:, :int(cols)], obs_names, var_names)
 
     @classmethod
     def upload(cls, tmap, dtype='float64', device='cuda'):
@@ -47,14 +75,15 @@ class DeviceMap:
         if hasattr(tmap, 'X') and hasattr(tmap, 'obs'):
             X = tmap.X
             obs, var = tmap.obs.index, tmap.var.index
-        if isinstance(X, torch.Tensor):
-            t = X.to(device=device, dtype=tdt)
-        else:
+        if not isinstance(X, torch.Tensor):
             if hasattr(X, 'toarray'):  # scipy sparse
                 X = X.toarray()
-            t = torch.from_numpy(np.ascontiguousarray(np.asarray(X, dtype=np.float64))).to(device=device,
-                                                                                         dtype=tdt)
-        return cls(t.contiguous(), obs, var)
+            X = torch.from_numpy(np.ascontiguousarray(np.asarray(X, dtype=np.float64)))
+        if X.ndim != 2:
+            raise ValueError('a transport map is 2-D, got shape %s' % (tuple(X.shape),))
+        out = cls.empty(X.shape[0], X.shape[1], dtype, device, obs, var)
+        out.X.copy_(X)
+        return out
 
     @property
     def shape(self):
@@ -67,7 +96,9 @@ class DeviceMap:
     def struct(self):
         torch = _dev.torch_mod()
         dt = _native.DTYPE_F64 if self.X.dtype == torch.float64 else _native.DTYPE_F32
-        return _native.TMap(self.X.data_ptr(), dt, 0, self.X.stride(0), self.X.shape[0], self.X.shape[1])
+        t = self.XT
+        return _native.TMap(self.X.data_ptr(), dt, 0, self.X.stride(0), self.X.shape[0], self.X.shape[1],
+                            t.data_ptr() if t is not None else None, t.stride(0) if t is not None else 0)
 
 
 def _cells_in(m, direction):
@@ -78,7 +109,7 @@ def _cells_out(m, direction):
     return m.shape[1] if direction == PUSH_FORWARD else m.shape[0]
 
 
-def propagate(maps, pops, direction, normalize=True, outs=None):
+def propagate(maps, pops, direction, normalize=True, outs=None, stats=None):
     """Push (``direction=PUSH_FORWARD``: p <- p T) or pull (``PULL_BACK``:
     p <- (T p^T)^T) the populations ``pops`` (device tensor, cells x k,
     float64) through ``maps`` (DeviceMaps, in the order they are applied).
@@ -87,6 +118,8 @@ def propagate(maps, pops, direction, normalize=True, outs=None):
     with one entry per map: a cells x k float64 device tensor (row stride >= k,
     unit column stride) receiving the populations after that map, or None
     (not kept; the last entry must be a tensor).  Returns the final tensor.
+    ``stats``: a dict that receives the library's kernel timer (maps applied,
+    summed product-kernel ms; the call then synchronises).
     """
     torch = _dev.torch_mod()
     maps = list(maps)
@@ -118,9 +151,12 @@ def propagate(maps, pops, direction, normalize=True, outs=None):
     nbytes = L.wotk_tmap_workspace_size(arr, len(maps), direction, k)
     ws = _dev.workspace(torch, nbytes, pops.device)
     ptrs = (ctypes.c_void_p * len(maps))(*[o.data_ptr() if o is not None else None for o in outs])
+    st = _native.Stats() if stats is not None else None
     _native.check(L.wotk_tmap_propagate(arr, len(maps), direction, _dev._ptr(pops), pops.stride(0), k,
                                         1 if normalize else 0, ptrs, ld, _dev._ptr(ws), ws.numel(),
-                                        _dev._stream(torch)))
+                                        ctypes.byref(st) if st is not None else None, _dev._stream(torch)))
+    if st is not None:
+        stats.update(maps=int(st.sweeps), kernel_ms=float(st.sweep_ms))
     return outs[-1]
 
 
