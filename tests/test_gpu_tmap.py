@@ -303,3 +303,34 @@ def test_model_from_time_course(tm):
     cp = model.get_coupling(1.0, 2.0)
     np.testing.assert_array_equal(cp.X, maps[1])
     assert list(cp.obs.index[:2]) == ['t1_0', 't1_1']
+
+
+def test_otmodel_h5ad_course_through_from_directory(tm, tmp_path):
+    """OTModel writes a 3-day course as h5ad (the default format, through the
+    HDF5 C library); TransportMapModel.from_directory reads ids and days from
+    the files and walks the maps on the device; results match the oracle on
+    the maps read back."""
+    import wot_amd.ot as ot
+    from wot_amd import _anndata, io
+    rng = np.random.default_rng(41)
+    sizes = [60, 80, 70]
+    X = np.vstack([rng.standard_normal((n, 5)) + 0.2 * t for t, n in enumerate(sizes)])
+    days = np.concatenate([[float(t)] * n for t, n in enumerate(sizes)])
+    obs = pd.DataFrame({'day': days}, index=['c%d' % i for i in range(len(days))])
+    adata = _anndata.AnnData(X, obs, pd.DataFrame(index=['f%d' % k for k in range(5)]))
+    model = ot.OTModel(adata, local_pca=0, solver='fixed_iters', scaling_iter=300, extra_iter=0)
+    out = str(tmp_path / 'tmaps')
+    model.compute_all_transport_maps(tmap_out=out)
+    paths = [out + '_0.0_1.0.h5ad', out + '_1.0_2.0.h5ad']
+    maps = [io.read_dataset(p).X for p in paths]
+    assert maps[0].shape == (60, 80) and maps[1].shape == (80, 70)
+    tmm = tm.TransportMapModel.from_directory(out)
+    assert tmm.timepoints == [0.0, 1.0, 2.0]
+    assert list(tmm.meta.index) == list(obs.index)
+    pops = tmm.population_from_cell_sets({'a': ['c%d' % i for i in range(60, 100)],
+                                          'b': ['c%d' % i for i in range(100, 140)]}, at_time=1)
+    P = np.vstack([p.p for p in pops])
+    traj = tmm.trajectories(pops)
+    assert rel(traj.X, O.trajectories(maps, P, 1)) < 1e-13
+    f = tmm.fates(pops)
+    assert rel(f.X, O.fates(maps, P, 1)) < 1e-13

@@ -1,7 +1,10 @@
-"""Transport-map output (the subset of wot/io/io.py:439-478 the OT driver uses).
+"""Transport-map files (the subset of wot/io/io.py:380-478 the OT driver and
+TransportMapModel use).
 
 'txt' (tab-separated, ``id`` index label) and 'npz' need nothing beyond
-numpy/pandas; 'h5ad' needs the anndata package.
+numpy/pandas; 'h5ad' is written and read through the HDF5 C library
+(``wot_amd.h5ad``) in AnnData's on-disk layout, or by the anndata package when
+it is installed.
 """
 import numpy as np
 import pandas as pd
@@ -21,9 +24,11 @@ def write_dataset(ds, path, output_format='txt'):
         pd.DataFrame(np.asarray(ds.X), index=ds.obs.index, columns=ds.var.index).to_csv(
             path, index_label='id', sep='\t', doublequote=False)
     elif output_format == 'h5ad':
-        if not hasattr(ds, 'write'):
-            raise ImportError("writing h5ad needs the anndata package; use output_file_format='npz' or 'txt'")
-        ds.write(path)
+        if hasattr(ds, 'write'):  # the anndata package
+            ds.write(path)
+        else:
+            from . import h5ad
+            h5ad.write_h5ad(path, np.asarray(ds.X), ds.obs, ds.var)
     elif output_format == 'npz':
         obs = {('obs_' + c): ds.obs[c].values for c in ds.obs.columns}
         np.savez(path, X=np.asarray(ds.X), obs_index=np.array([str(v) for v in ds.obs.index], dtype=str),
@@ -42,12 +47,15 @@ def read_npz_tmap(path):
 
 
 def read_dataset(path):
-    """Read a transport map written by write_dataset ('txt' / 'csv' with an
-    'id' index column, or 'npz') as AnnData (the map formats of
-    wot/io/io.py:read_dataset that need no optional package)."""
+    """Read a transport map written by write_dataset ('h5ad', 'txt' / 'csv'
+    with an 'id' index column, or 'npz') as AnnData (the map formats of
+    wot/io/io.py:380-430 read_dataset)."""
     from . import _anndata
     p = str(path)
     low = p.lower()
+    if low.endswith('.h5ad'):
+        from . import h5ad
+        return h5ad.read_h5ad(p)
     if low.endswith('.npz'):
         X, obs, var = read_npz_tmap(p)
         return _anndata.AnnData(np.asarray(X, dtype=np.float64), obs, pd.DataFrame(index=var))
@@ -56,4 +64,4 @@ def read_dataset(path):
         df = pd.read_csv(p, sep=sep, index_col=0)
         return _anndata.AnnData(df.values.astype(np.float64), pd.DataFrame(index=df.index.astype(str)),
                                 pd.DataFrame(index=df.columns.astype(str)))
-    raise ValueError('Unknown file format for %s (supported: .npz, .txt, .tsv, .csv)' % p)
+    raise ValueError('Unknown file format for %s (supported: .h5ad, .npz, .txt, .tsv, .csv)' % p)

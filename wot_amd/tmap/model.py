@@ -73,6 +73,87 @@ class TransportMapModel:
             tmaps[(t0, t1)] = dm
         return cls(tmaps, meta, timepoints=list(days), dtype=dtype, transposed_maps=transposed_maps)
 
+    # ------------------------------------------------------------------
+    # index files and map directories (transport_map_model.py:617-732)
+    # ------------------------------------------------------------------
+    def to_json(self, path):
+        """Day pairs, map paths, time points and cell days as JSON (:617-628)."""
+        import json
+        meta = self.meta.to_dict(orient='list')
+        meta['id'] = self.meta.index.values.tolist()
+        paths = [self.tmaps[tuple(dp)] for dp in self.day_pairs]
+        d = {'day_pairs': list(self.day_pairs), 'paths': paths, 'timepoints': list(self.timepoints), 'meta': meta}
+        with open(path, 'w') as f:
+            json.dump(d, f, ensure_ascii=False)
+
+    @staticmethod
+    def from_json(index_path, **kw):
+        """Model from a to_json index (:630-649); maps stay paths until first use."""
+        import json
+        if str(index_path).startswith('gs://'):
+            raise ValueError('gs:// index files need google-cloud-storage; download the index first')
+        with open(index_path, 'r') as r:
+            info = json.load(r)
+        meta = pd.DataFrame(index=info['meta']['id'], data={'day': info['meta']['day']})
+        day_pairs = info['day_pairs']
+        tmaps = {tuple(day_pairs[i]): info['paths'][i] for i in range(len(day_pairs))}
+        return TransportMapModel(tmaps=tmaps, meta=meta, timepoints=info['timepoints'], day_pairs=day_pairs, **kw)
+
+    @staticmethod
+    def from_directory(tmap_out, with_covariates=False, cache=False, **kw):
+        """Model over the maps ``<prefix>_<t0>_<t1>.h5ad`` in a directory
+        (:651-732): cell ids and days come from the maps' /obs and /var
+        indices (the target ids of the last pair only), without reading X."""
+        import os
+        import re
+        from .. import h5ad
+        tmap_dir, tmap_prefix = os.path.split(tmap_out)
+        tmap_dir = tmap_dir or '.'
+        tmap_prefix = tmap_prefix or 'tmaps'
+        day = r'([0-9]*\.?[0-9]+)'
+        # the reference's suffix is a character class ('[\.h5ad|\.loom]'): one of those characters
+        if with_covariates:
+            pattern = re.compile(tmap_prefix + r'_{}_{}_cv([a-zA-Z0-9]+)_cv([a-zA-Z0-9]+)[\.h5ad|\.loom]'.format(day, day))
+        else:
+            pattern = re.compile(tmap_prefix + r'_{}_{}[\.h5ad|\.loom]'.format(day, day))
+        tmaps = {}
+        for f in os.listdir(tmap_dir):
+            path = os.path.join(tmap_dir, f)
+            if not os.path.isfile(path):
+                continue
+            m = pattern.match(f)
+            if m is None:
+                continue
+            try:
+                t1, t2 = float(m.group(1)), float(m.group(2))
+            except ValueError:
+                print('Unable to find day pair for ' + f)
+                continue
+            if with_covariates:
+                tmaps[(t1, t2, m.group(3), m.group(4))] = path
+            else:
+                tmaps[(t1, t2)] = path
+        if len(tmaps) == 0:
+            raise ValueError('No transport maps found in ' + tmap_dir + ' with prefix ' + tmap_prefix)
+        day_pairs, timepoints = set(), set()
+        keys = sorted(tmaps.keys(), key=lambda x: x[0])
+        parts = []
+        for i, key in enumerate(keys):
+            t0, t1 = key[0], key[1]
+            day_pairs.add((t0, t1))
+            timepoints.add(t0)
+            timepoints.add(t1)
+            if not with_covariates:
+                path = tmaps[key]
+                if path.endswith('.loom'):
+                    raise ValueError('loom transport maps are not supported here: ' + path)
+                parts.append(pd.DataFrame(index=h5ad.read_index(path, 'obs'), data={'day': t0}))
+                if i == len(keys) - 1:
+                    parts.append(pd.DataFrame(index=h5ad.read_index(path, 'var'), data={'day': t1}))
+        meta = pd.concat(parts, copy=False) if parts else None
+        return TransportMapModel(tmaps=tmaps, meta=meta, timepoints=sorted(timepoints), day_pairs=day_pairs,
+                                 cache=cache, **kw)
+
     def device_map(self, t0, t1):
         """The map (t0, t1) on the device, uploaded on first use and kept."""
         key = (t0, t1)
