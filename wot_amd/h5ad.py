@@ -215,10 +215,11 @@ def _write_array(loc, name, arr):
             buf = arr.ctypes.data_as(_P)
             key = None
         elif arr.dtype.kind == 'f':
-            arr = arr.astype(np.float32 if arr.dtype == np.float32 else np.float64)
+            arr = arr.astype(np.float32 if arr.dtype == np.float32 else np.float64, copy=False)
             key = 'f32' if arr.dtype == np.float32 else 'f64'
         elif arr.dtype.kind in 'iu':
-            arr = arr.astype(np.int64) if arr.dtype.itemsize > 4 or arr.dtype.kind == 'u' else arr.astype(np.int32)
+            wide = arr.dtype.itemsize > 4 or arr.dtype.kind == 'u'
+            arr = arr.astype(np.int64 if wide else np.int32, copy=False)
             key = 'i64' if arr.dtype == np.int64 else 'i32'
         else:
             raise TypeError('unsupported column dtype %s' % arr.dtype)
