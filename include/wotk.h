@@ -290,6 +290,21 @@ int wotk_tmap_transpose(const void *src, int32_t dtype, int64_t ld_src, int64_t 
 int wotk_rows_normalize_f64(double *X, int64_t n, int32_t k, int64_t ld, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Local PCA (wot/ot/util.py:240-255 compute_pca, used by ot_model.py:296-301):
+ * the stacked cells of both days x (n cells x g genes, FP64 or FP32 by dtype,
+ * leading dimension ldx) centered per gene and then per cell (sklearn's PCA is
+ * fitted on x^T and centers its columns):
+ *   A[c][j] = x[c][j] - gene_mean[j] - cell_mean[c]
+ * gene_mean (g) is the reference's mean_shift; cell_mean (n) is the per-cell
+ * mean of the gene-centered data.  A (n x g, lda, FP64) may be NULL (means
+ * only).  The singular vectors of A give the PCA coordinates.
+ * ------------------------------------------------------------------------- */
+size_t wotk_pca_center_workspace_size(int64_t n, int64_t g);
+int wotk_pca_center(const void *x, int32_t dtype, int64_t ldx, int64_t n, int64_t g, double *A,
+                    int64_t lda, double *gene_mean, double *cell_mean, void *workspace,
+                    size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Tile shapes of the sweep kernels (no counterpart in the reference, which has
  * no device kernels).  A tile is (rows of the cost matrix per workgroup,
  * threads per workgroup).  The library keeps one tile per (kernel, shape

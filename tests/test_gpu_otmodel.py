@@ -84,7 +84,8 @@ def test_otmodel_coordinates_growth_loop(amd, tmp_path):
 
 
 def test_otmodel_local_pca_duality_gap(amd):
-    """Default path: local PCA (sklearn, as the reference) + duality-gap solver."""
+    """Default path: local PCA (device) + duality-gap solver, against the
+    oracle pipeline with the reference's own sklearn PCA call."""
     ot, _anndata = amd
     rng = np.random.default_rng(3)
     genes = 200
@@ -93,7 +94,8 @@ def test_otmodel_local_pca_duality_gap(amd):
     adata = _adata(_anndata, X, [1.0] * 120 + [2.0] * 140)
     model = ot.OTModel(adata)
     tmap = model.compute_transport_map(1.0, 2.0)
-    p0, p1, pca, _ = ot.compute_pca(X[:120], X[120:], 30)
+    from oracle import pca_oracle
+    p0, p1, pca, _ = pca_oracle.reference_pca(X[:120], X[120:], 30)
     eig = np.diag(pca.singular_values_)
     C = O.default_cost_matrix(p0, p1, eig)
     R = O.optimal_transport_duality_gap(C, np.ones(120), **model.ot_config)

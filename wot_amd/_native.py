@@ -105,6 +105,8 @@ SIGNATURES = {
                                            ctypes.POINTER(_P), _I64, _P, _SZ, ctypes.POINTER(Stats), _P]),
     'wotk_rows_normalize_f64': (ctypes.c_int, [_P, _I64, _I32, _I64, _P]),
     'wotk_tmap_transpose': (ctypes.c_int, [_P, _I32, _I64, _I64, _I64, _P, _I64, _P]),
+    'wotk_pca_center_workspace_size': (_SZ, [_I64, _I64]),
+    'wotk_pca_center': (ctypes.c_int, [_P, _I32, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _SZ, _P]),
     'wotk_tile_variants': (_I32, [_I32, ctypes.POINTER(Tile), _I32]),
     'wotk_shape_bucket': (_I32, [_I32, _I64, _I64, _I32]),
     'wotk_num_buckets': (_I32, [_I32]),

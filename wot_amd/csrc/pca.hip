@@ -1,0 +1,172 @@
+// Double centering for local PCA (wot/ot/util.py:240-255 compute_pca): the
+// cells of both days are stacked (x: n cells x g genes), centered per gene
+// (x - mean over cells, util.py:244-246) and PCA is fitted on x^T, which
+// sklearn centers once more per column of x^T, i.e. per cell over genes.
+// The matrix whose singular vectors the PCA needs is therefore
+//   A[c][j] = x[c][j] - mu_j - nu_c,  mu_j = mean_c x[c][j],
+//   nu_c = mean_j (x[c][j] - mu_j) = mean_j x[c][j] - mean_j mu_j.
+// Two streaming passes: partial column and row sums per (row block, column
+// block) tile (fixed order: deterministic), then A written in one pass.  HBM
+// bound: 2 reads of x + 1 write of A.
+#include "common.h"
+
+namespace wotk {
+namespace {
+
+constexpr int kTileRows = 64;
+constexpr int kTileCols = 256;
+
+// partial sums of one 64 x 256 tile: colpart[rb][j] (over the tile's rows),
+// rowpart[cb][c] (over the tile's columns)
+template <typename T>
+__global__ __launch_bounds__(256) void center_sums_kernel(const T *__restrict__ x, int64_t ldx, int64_t n,
+                                                          int64_t g, double *__restrict__ colpart,
+                                                          double *__restrict__ rowpart) {
+    __shared__ double rs[kTileRows][17];
+    const int rb = blockIdx.y, cb = blockIdx.x;
+    const int64_t r0 = (int64_t)rb * kTileRows, c0 = (int64_t)cb * kTileCols;
+    const int t = threadIdx.x;
+    const int64_t j = c0 + t;
+    // column sums: thread t owns column c0 + t, walks the 64 rows (coalesced per row)
+    double cs = 0.0;
+    for (int r = 0; r < kTileRows; ++r) {
+        const int64_t row = r0 + r;
+        if (row < n && j < g) cs += (double)x[row * ldx + j];
+    }
+    if (j < g) colpart[(int64_t)rb * g + j] = cs;
+    // row sums: 16 threads per row (columns t%16, t%16 + 16, ...), 16 rows per pass
+    const int lane = t & 15, rr = t >> 4;
+    for (int base = 0; base < kTileRows; base += 16) {
+        const int64_t row = r0 + base + rr;
+        double s = 0.0;
+        if (row < n)
+            for (int c = lane; c < kTileCols; c += 16) {
+                const int64_t col = c0 + c;
+                if (col < g) s += (double)x[row * ldx + col];
+            }
+        rs[base + rr][lane] = s;
+    }
+    __syncthreads();
+    if (t < kTileRows) {
+        const int64_t row = r0 + t;
+        double s = 0.0;
+        for (int l = 0; l < 16; ++l) s += rs[t][l];
+        if (row < n) rowpart[(int64_t)cb * n + row] = s;
+    }
+}
+
+// mu_j = sum_rb colpart[rb][j] / n
+__global__ __launch_bounds__(256) void center_colmean_kernel(const double *__restrict__ colpart, int n_rb, int64_t n,
+                                                             int64_t g, double *__restrict__ mu) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= g) return;
+    double s = 0.0;
+    for (int rb = 0; rb < n_rb; ++rb) s += colpart[(int64_t)rb * g + j];
+    mu[j] = s / (double)n;
+}
+
+// mean of mu (one workgroup, fixed order)
+__global__ __launch_bounds__(256) void center_mumean_kernel(const double *__restrict__ mu, int64_t g,
+                                                            double *__restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t j = threadIdx.x; j < g; j += 256) s += mu[j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] / (double)g;
+}
+
+// nu_c = sum_cb rowpart[cb][c] / g - mean(mu)
+__global__ __launch_bounds__(256) void center_rowmean_kernel(const double *__restrict__ rowpart, int n_cb, int64_t n,
+                                                             int64_t g, const double *__restrict__ mumean,
+                                                             double *__restrict__ nu) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    double s = 0.0;
+    for (int cb = 0; cb < n_cb; ++cb) s += rowpart[(int64_t)cb * n + c];
+    nu[c] = s / (double)g - mumean[0];
+}
+
+// A[c][j] = x[c][j] - mu_j - nu_c
+template <typename T>
+__global__ __launch_bounds__(256) void center_apply_kernel(const T *__restrict__ x, int64_t ldx, int64_t n, int64_t g,
+                                                           const double *__restrict__ mu,
+                                                           const double *__restrict__ nu, double *__restrict__ A,
+                                                           int64_t lda) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.y * kTileRows;
+    if (j >= g) return;
+    const double m = mu[j];
+    for (int r = 0; r < kTileRows; ++r) {
+        const int64_t row = r0 + r;
+        if (row >= n) break;
+        A[row * lda + j] = ((double)x[row * ldx + j] - m) - nu[row];
+    }
+}
+
+struct CenterLayout {
+    double *colpart, *rowpart, *mumean;
+};
+
+CenterLayout center_carve(Arena &ar, int64_t n, int64_t g) {
+    const int64_t n_rb = (n + kTileRows - 1) / kTileRows, n_cb = (g + kTileCols - 1) / kTileCols;
+    CenterLayout L;
+    L.colpart = ar.take<double>((size_t)(n_rb * g));
+    L.rowpart = ar.take<double>((size_t)(n_cb * n));
+    L.mumean = ar.take<double>(1);
+    return L;
+}
+
+template <typename T>
+int center_impl(const T *x, int64_t ldx, int64_t n, int64_t g, double *A, int64_t lda, double *mu, double *nu,
+                const CenterLayout &L, hipStream_t st) {
+    const int n_rb = (int)((n + kTileRows - 1) / kTileRows), n_cb = (int)((g + kTileCols - 1) / kTileCols);
+    center_sums_kernel<T><<<dim3(n_cb, n_rb), 256, 0, st>>>(x, ldx, n, g, L.colpart, L.rowpart);
+    WOTK_LAUNCH_CHECK();
+    center_colmean_kernel<<<(unsigned)((g + 255) / 256), 256, 0, st>>>(L.colpart, n_rb, n, g, mu);
+    WOTK_LAUNCH_CHECK();
+    center_mumean_kernel<<<1, 256, 0, st>>>(mu, g, L.mumean);
+    WOTK_LAUNCH_CHECK();
+    center_rowmean_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(L.rowpart, n_cb, n, g, L.mumean, nu);
+    WOTK_LAUNCH_CHECK();
+    if (A) {
+        center_apply_kernel<T><<<dim3((unsigned)((g + 255) / 256), n_rb), 256, 0, st>>>(x, ldx, n, g, mu, nu, A,
+                                                                                      lda);
+        WOTK_LAUNCH_CHECK();
+    }
+    return WOTK_OK;
+}
+
+}  // namespace
+}  // namespace wotk
+
+using namespace wotk;
+
+extern "C" size_t wotk_pca_center_workspace_size(int64_t n, int64_t g) {
+    if (n <= 0 || g <= 0) return 0;
+    Arena ar(nullptr, 0);
+    center_carve(ar, n, g);
+    return ar.off;
+}
+
+extern "C" int wotk_pca_center(const void *x, int32_t dtype, int64_t ldx, int64_t n, int64_t g, double *A,
+                               int64_t lda, double *gene_mean, double *cell_mean, void *workspace,
+                               size_t workspace_bytes, void *stream) {
+    WOTK_REQUIRE(x && n > 0 && g > 0 && ldx >= g, "pca_center: bad matrix (%lld x %lld, ld %lld)", (long long)n,
+                 (long long)g, (long long)ldx);
+    WOTK_REQUIRE(dtype == WOTK_DTYPE_F64 || dtype == WOTK_DTYPE_F32, "pca_center: bad dtype %d", dtype);
+    WOTK_REQUIRE(gene_mean && cell_mean, "pca_center: gene_mean and cell_mean are required");
+    WOTK_REQUIRE(!A || lda >= g, "pca_center: lda %lld < %lld", (long long)lda, (long long)g);
+    WOTK_REQUIRE((n + kTileRows - 1) / kTileRows < 65536, "pca_center: too many cells (%lld)", (long long)n);
+    Arena ar(workspace, workspace_bytes);
+    CenterLayout L = center_carve(ar, n, g);
+    WOTK_REQUIRE(ar.ok && workspace, "pca_center: workspace too small (%zu < %zu bytes)", workspace_bytes, ar.off);
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == WOTK_DTYPE_F64)
+        return center_impl(static_cast<const double *>(x), ldx, n, g, A, lda, gene_mean, cell_mean, L, st);
+    return center_impl(static_cast<const float *>(x), ldx, n, g, A, lda, gene_mean, cell_mean, L, st);
+}

@@ -17,7 +17,7 @@ from .. import _anndata
 from . import device as _dev
 from . import optimal_transport as _ot
 from .initializer import parse_configuration, parse_parameter_file
-from .util import compute_pca
+from .pca import compute_pca
 
 logger = logging.getLogger('wot')
 
