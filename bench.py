@@ -681,6 +681,73 @@ def _tmap_cpu_baseline(counts, k, total_bytes, gpu_per):
             'GBps': rate / 1e9, 'gpu_over_cpu': (1.0 / gpu_per) / (rate / total_bytes)}
 
 
+PCA_CFG = dict(n_cells=(15000, 15000), genes=2000, rank=40, k=30, seed=11)
+
+
+def bench_pca(args):
+    """Local PCA of one day pair (OTModel.compute_single_transport_map with
+    local_pca=30, ot_model.py:296-301): 15,000 + 15,000 cells x 2,000 genes
+    (FP32, low-rank signal + noise), k = 30.  value = PCA fits per second
+    with the expression matrix resident on the device; roofline on the
+    double-centering kernel (2 reads of x + 1 FP64 write of A).  Baseline:
+    the reference's own call (sklearn PCA 'auto', random_state 58951) on the
+    host cores."""
+    import torch
+    from wot_amd.ot import pca as gpca
+    from oracle import pca_oracle as P
+    n1, n2 = PCA_CFG['n_cells']
+    g, r, k = PCA_CFG['genes'], PCA_CFG['rank'], PCA_CFG['k']
+    rng = np.random.default_rng(PCA_CFG['seed'])
+    W = rng.standard_normal((r, g)).astype(np.float32)
+    a = (rng.standard_normal((n1, r)).astype(np.float32) @ W + 0.3 * rng.standard_normal((n1, g)).astype(np.float32))
+    b = (rng.standard_normal((n2, r)).astype(np.float32) @ W + 0.3 * rng.standard_normal((n2, g)).astype(np.float32)
+         + 0.2)
+    x = torch.from_numpy(np.vstack([a, b])).cuda()
+    for _ in range(max(args.warmup, 1)):
+        gpca.fit(x, k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = gpca.fit(x, k)
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t0) / args.steps
+    # centering kernel alone, timed with events on the current stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    reps = 10
+    for _ in range(reps):
+        gpca.center(x)
+    ev[1].record()
+    torch.cuda.synchronize()
+    c_us = ev[0].elapsed_time(ev[1]) * 1000.0 / reps
+    n = n1 + n2
+    alg = 2.0 * 4 * n * g + 8.0 * n * g + 8.0 * (n + g)
+    out = {'metric': 'local PCA fits/sec (one day pair, k=30)', 'value': 1.0 / per, 'unit': 'fits/s', 'n_gpus': 1,
+           'steps': args.steps, 'warmup': max(args.warmup, 1), 'ms_per_step': 1000.0 * per, 'higher_is_better': True,
+           'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32 in / f64', 'data': 'synthetic (rank-40 + noise)',
+           'config': {'workload': 'compute_pca, %d + %d cells x %d genes, k=%d, exact Gram method' % (n1, n2, g, k)},
+           'roofline': {'bound': 'hbm', 'kernel': 'center_sums_kernel + center_apply_kernel', 'avg_launch_us': c_us,
+                        'achieved': alg / (c_us * 1e-6) / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                        'frac': alg / (c_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 'traffic': None}}
+    if not args.no_baselines:
+        import os as _os
+        t0 = time.perf_counter()
+        r1, r2, rp, _ = P.reference_pca(a, b, k)
+        cpu = time.perf_counter() - t0
+        Z = res.coordinates().cpu().numpy()
+        R = np.vstack([r1, r2]) * rp.singular_values_
+        sel = rng.choice(n, 2000, replace=False)
+        d_gpu, d_ref = P.sqdist(Z[sel]), P.sqdist(R[sel].astype(np.float64))
+        out['parity_vs_reference_sklearn'] = {
+            'singular_values_rel': float(np.linalg.norm(res.singular_values_ - rp.singular_values_)
+                                         / np.linalg.norm(rp.singular_values_)),
+            'sqdist_rel_2000_cells': float(np.linalg.norm(d_gpu - d_ref) / np.linalg.norm(d_ref))}
+        out['cpu_baseline'] = {'value': 1.0 / cpu, 'unit': 'fits/s', 'cores': _os.cpu_count(), 'kind': 'reference',
+                               'sample': "the reference's call: sklearn PCA(30, random_state=58951).fit(x.T) on the "
+                                         'same FP32 matrix, one fit'}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -690,10 +757,11 @@ def main():
     ap.add_argument('--large-n', type=int, default=60000, help='large only: cells per day')
     ap.add_argument('--group-size', type=int, default=None,
                     help='timecourse only: combinations per batched run (default: the whole grid)')
-    ap.add_argument('--config', default='pair', choices=['pair', 'timecourse', 'adjoint', 'large', 'tmap'],
+    ap.add_argument('--config', default='pair', choices=['pair', 'timecourse', 'adjoint', 'large', 'tmap', 'pca'],
                     help='pair: one 15k x 15k map (default); timecourse: 38 pairs x 9-point grid; '
                          'adjoint: gradient of the converged objective, 8k x 8k; '
-                         'tmap: trajectories through the 38 maps of the time course')
+                         'tmap: trajectories through the 38 maps of the time course; '
+                         'pca: local PCA of one 15k + 15k cell day pair')
     ap.add_argument('--tmap-dtype', default='float64', choices=['float64', 'float32'],
                     help='tmap only: precision of the stored maps')
     ap.add_argument('--cost-mode', default='bf16x6', choices=['f64', 'bf16x3', 'bf16x6'],
@@ -718,6 +786,9 @@ def main():
         return
     if args.config == 'tmap':
         print(json.dumps(bench_tmap(args)), flush=True)
+        return
+    if args.config == 'pca':
+        print(json.dumps(bench_pca(args)), flush=True)
         return
 
     import torch

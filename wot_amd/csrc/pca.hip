@@ -55,14 +55,19 @@ __global__ __launch_bounds__(256) void center_sums_kernel(const T *__restrict__ 
     }
 }
 
-// mu_j = sum_rb colpart[rb][j] / n
+// mu_j = sum_rb colpart[rb][j] / n: 64 columns per workgroup, 4 threads per
+// column each summing every 4th row block, combined in a fixed order
 __global__ __launch_bounds__(256) void center_colmean_kernel(const double *__restrict__ colpart, int n_rb, int64_t n,
                                                              int64_t g, double *__restrict__ mu) {
-    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (j >= g) return;
+    __shared__ double red[4][64];
+    const int c = threadIdx.x & 63, part = threadIdx.x >> 6;
+    const int64_t j = (int64_t)blockIdx.x * 64 + c;
     double s = 0.0;
-    for (int rb = 0; rb < n_rb; ++rb) s += colpart[(int64_t)rb * g + j];
-    mu[j] = s / (double)n;
+    if (j < g)
+        for (int rb = part; rb < n_rb; rb += 4) s += colpart[(int64_t)rb * g + j];
+    red[part][c] = s;
+    __syncthreads();
+    if (part == 0 && j < g) mu[j] = (((red[0][c] + red[1][c]) + red[2][c]) + red[3][c]) / (double)n;
 }
 
 // mean of mu (one workgroup, fixed order)
@@ -127,7 +132,7 @@ int center_impl(const T *x, int64_t ldx, int64_t n, int64_t g, double *A, int64_
     const int n_rb = (int)((n + kTileRows - 1) / kTileRows), n_cb = (int)((g + kTileCols - 1) / kTileCols);
     center_sums_kernel<T><<<dim3(n_cb, n_rb), 256, 0, st>>>(x, ldx, n, g, L.colpart, L.rowpart);
     WOTK_LAUNCH_CHECK();
-    center_colmean_kernel<<<(unsigned)((g + 255) / 256), 256, 0, st>>>(L.colpart, n_rb, n, g, mu);
+    center_colmean_kernel<<<(unsigned)((g + 63) / 64), 256, 0, st>>>(L.colpart, n_rb, n, g, mu);
     WOTK_LAUNCH_CHECK();
     center_mumean_kernel<<<1, 256, 0, st>>>(mu, g, L.mumean);
     WOTK_LAUNCH_CHECK();
