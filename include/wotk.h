@@ -305,6 +305,39 @@ int wotk_pca_center(const void *x, int32_t dtype, int64_t ldx, int64_t n, int64_
                     size_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Interpolated populations for OT validation (wot/ot/util.py:109-200).
+ * wotk_interp_weights: W (I x J, contiguous) = T / colsum(T)^(1 - frac)
+ *   (util.py:142; colsum (J) receives the column sums).
+ * wotk_interp_sample: idx[s] = first k < n with cdf[k] > u[s] (numpy's
+ *   cdf.searchsorted(u, side='right') inside RandomState.choice); u are the
+ *   caller's uniforms, so a caller drawing them from numpy's stream gets the
+ *   reference's samples.
+ * wotk_interp_gather: out[s] = p0[idx/J] (1 - frac) + p1[idx%J] frac (d columns).
+ * ------------------------------------------------------------------------- */
+int wotk_interp_weights(const double *T, int64_t ld, int64_t I, int64_t J, double frac, double *W,
+                        double *colsum, void *stream);
+int wotk_interp_sample(const double *cdf, int64_t n, const double *u, int64_t size, int64_t *idx,
+                       void *stream);
+int wotk_interp_gather(const int64_t *idx, int64_t size, const double *p0, int64_t ld0, int64_t I,
+                       const double *p1, int64_t ld1, int64_t J, int64_t d, double frac, double *out,
+                       int64_t ldo, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Exact earth mover's distance (POT's emd2 as wot/ot/util.py:202-237 calls it,
+ * numItermax): min sum_ij F_ij C_ij over F >= 0 with row sums a (n) and column
+ * sums b (m), by a primal network simplex.  HOST pointers (the only host-side
+ * entry point: exact pivoting is sequential; the device builds C).  C is
+ * n x m with leading dimension ldc; flow_out (n x m, may be NULL) receives the
+ * optimal plan; max_iter <= 0: no limit.  *status_out: 0 optimal, 1 stopped at
+ * max_iter (the plan so far: part of the mass may still sit on the artificial
+ * start arcs, so its marginals can fall short).  a and b must be
+ * non-negative with equal sums (relative 1e-9).
+ * ------------------------------------------------------------------------- */
+int wotk_emd(const double *a, int64_t n, const double *b, int64_t m, const double *C, int64_t ldc,
+             int64_t max_iter, double *cost_out, double *flow_out, int64_t *iterations_out,
+             int32_t *status_out);
+
+/* ---------------------------------------------------------------------------
  * Tile shapes of the sweep kernels (no counterpart in the reference, which has
  * no device kernels).  A tile is (rows of the cost matrix per workgroup,
  * threads per workgroup).  The library keeps one tile per (kernel, shape

@@ -106,6 +106,11 @@ SIGNATURES = {
     'wotk_rows_normalize_f64': (ctypes.c_int, [_P, _I64, _I32, _I64, _P]),
     'wotk_tmap_transpose': (ctypes.c_int, [_P, _I32, _I64, _I64, _I64, _P, _I64, _P]),
     'wotk_pca_center_workspace_size': (_SZ, [_I64, _I64]),
+    'wotk_interp_weights': (ctypes.c_int, [_P, _I64, _I64, _I64, _D, _P, _P, _P]),
+    'wotk_interp_sample': (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P]),
+    'wotk_interp_gather': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _P, _I64, _I64, _I64, _D, _P, _I64, _P]),
+    'wotk_emd': (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _I64, ctypes.POINTER(ctypes.c_double), _P,
+                                ctypes.POINTER(_I64), ctypes.POINTER(_I32)]),
     'wotk_pca_center': (ctypes.c_int, [_P, _I32, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _SZ, _P]),
     'wotk_tile_variants': (_I32, [_I32, ctypes.POINTER(Tile), _I32]),
     'wotk_shape_bucket': (_I32, [_I32, _I64, _I64, _I32]),
