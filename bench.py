@@ -748,6 +748,102 @@ def bench_pca(args):
     return out
 
 
+VAL_CFG = dict(cells=3000, genes=500, days=3, interp_size=3000, local_pca=30, seed=13)
+
+
+def bench_validation(args):
+    """compute_validation_summary (optimal_transport_validation.py:15-196) on
+    one day triplet: 3 x 3,000 cells x 500 genes (drifting low-rank signal,
+    growth rates, one covariate), local_pca = 30, interp_size = 3,000: local
+    PCA + middle-day projection, the (t0, t1) map, three interpolated
+    populations (I, R, Rg) and five exact EMDs (I, R, Rg, F, L).  value =
+    summaries per second.  The EMDs (network simplex) run on the host; the
+    device does PCA, maps, sampling and costs.  The roofline covers the
+    sampling-weight kernels (T read, W written)."""
+    import torch
+    import pandas as pd
+    import wot_amd.ot as ot
+    from wot_amd import _anndata
+    from wot_amd.ot import validation, emd as emd_mod
+    from oracle import validation_oracle as V
+    c = VAL_CFG
+    rng = np.random.default_rng(c['seed'])
+    W = rng.standard_normal((8, c['genes']))
+    blocks = [rng.standard_normal((c['cells'], 8)) @ W + 0.3 * t + 0.2 * rng.standard_normal((c['cells'], c['genes']))
+              for t in range(c['days'])]
+    X = np.vstack(blocks)
+    days = np.repeat(np.arange(c['days'], dtype=np.float64), c['cells'])
+    obs = pd.DataFrame({'day': days, 'cell_growth_rate': np.exp(0.2 * rng.standard_normal(len(days)))},
+                       index=['c%d' % i for i in range(len(days))])
+    adata = _anndata.AnnData(X, obs, pd.DataFrame(index=['g%d' % k for k in range(c['genes'])]))
+    model = ot.OTModel(adata, local_pca=c['local_pca'])
+    np.random.seed(0)
+    validation.compute_validation_summary(model, interp_size=200)  # warm-up (small)
+    torch.cuda.synchronize()
+    t_emd = [0.0]
+    real_emd = emd_mod.emd2
+
+    def timed_emd(*a, **k):
+        t = time.perf_counter()
+        r = real_emd(*a, **k)
+        t_emd[0] += time.perf_counter() - t
+        return r
+    emd_mod.emd2 = timed_emd
+    try:
+        np.random.seed(1)
+        t0 = time.perf_counter()
+        df = validation.compute_validation_summary(model, interp_size=c['interp_size'])
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+    finally:
+        emd_mod.emd2 = real_emd
+    # sampling-weight kernels on one 3000 x 3000 map, hipEvents on the current stream
+    tmap = model.compute_transport_map(0.0, 2.0, covariate=(1, 1)).X
+    Td = torch.from_numpy(tmap).cuda()
+    Wd = torch.empty_like(Td)
+    cs = torch.empty(Td.shape[1], dtype=torch.float64, device='cuda')
+    from wot_amd import _native
+    from wot_amd.ot import device as dev
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(10):
+        _native.check(_native.lib().wotk_interp_weights(dev._ptr(Td), Td.stride(0), Td.shape[0], Td.shape[1], 0.5,
+                                                        dev._ptr(Wd), dev._ptr(cs), dev._stream(torch)))
+    ev[1].record()
+    torch.cuda.synchronize()
+    w_us = ev[0].elapsed_time(ev[1]) * 100.0
+    alg = 3.0 * 8 * Td.numel()  # T read twice (column sums, weights), W written
+    out = {'metric': 'validation summaries/sec (one day triplet, 5 exact EMDs)', 'value': 1.0 / elapsed,
+           'unit': 'summaries/s', 'n_gpus': 1, 'steps': 1, 'warmup': 1, 'ms_per_step': 1000.0 * elapsed,
+           'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+           'data': 'synthetic (drifting rank-8 signal, 3 days)',
+           'config': {'workload': 'compute_validation_summary, 3 x %d cells x %d genes, local_pca=%d, interp_size=%d'
+                                  % (c['cells'], c['genes'], c['local_pca'], c['interp_size']),
+                      'rows': list(df['name'])},
+           'emd_host_s': t_emd[0], 'device_and_rest_s': elapsed - t_emd[0],
+           'roofline': {'bound': 'hbm', 'kernel': 'colsum_partial_kernel + colsum_final_kernel + weights_kernel',
+                        'avg_launch_us': w_us,
+                        'achieved': alg / (w_us * 1e-6) / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                        'frac': alg / (w_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 'traffic': None}}
+    if not args.no_baselines:
+        p0 = rng.standard_normal((3000, 30))
+        p1 = rng.standard_normal((3000, 30))
+        t = time.perf_counter()
+        np.random.seed(2)
+        V.interpolate_with_ot(p0, p1, tmap, 0.5, c['interp_size'])
+        cpu = time.perf_counter() - t
+        t = time.perf_counter()
+        np.random.seed(2)
+        validation.interpolate_with_ot(p0, p1, tmap, 0.5, c['interp_size'])
+        gpu = time.perf_counter() - t
+        out['cpu_baseline'] = {'value': None, 'unit': 'summaries/s', 'cores': 1, 'kind': 'port',
+                               'sample': 'interpolate_with_ot on one 3000 x 3000 map, %d samples: numpy %.3f s, '
+                                         'device %.3f s (the EMDs run on the host in both)' % (c['interp_size'], cpu,
+                                                                                                 gpu),
+                               'interp_cpu_s': cpu, 'interp_gpu_s': gpu}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -757,11 +853,12 @@ def main():
     ap.add_argument('--large-n', type=int, default=60000, help='large only: cells per day')
     ap.add_argument('--group-size', type=int, default=None,
                     help='timecourse only: combinations per batched run (default: the whole grid)')
-    ap.add_argument('--config', default='pair', choices=['pair', 'timecourse', 'adjoint', 'large', 'tmap', 'pca'],
+    ap.add_argument('--config', default='pair', choices=['pair', 'timecourse', 'adjoint', 'large', 'tmap', 'pca', 'validation'],
                     help='pair: one 15k x 15k map (default); timecourse: 38 pairs x 9-point grid; '
                          'adjoint: gradient of the converged objective, 8k x 8k; '
                          'tmap: trajectories through the 38 maps of the time course; '
-                         'pca: local PCA of one 15k + 15k cell day pair')
+                         'pca: local PCA of one 15k + 15k cell day pair; '
+                         'validation: compute_validation_summary on one day triplet')
     ap.add_argument('--tmap-dtype', default='float64', choices=['float64', 'float32'],
                     help='tmap only: precision of the stored maps')
     ap.add_argument('--cost-mode', default='bf16x6', choices=['f64', 'bf16x3', 'bf16x6'],
@@ -789,6 +886,9 @@ def main():
         return
     if args.config == 'pca':
         print(json.dumps(bench_pca(args)), flush=True)
+        return
+    if args.config == 'validation':
+        print(json.dumps(bench_validation(args)), flush=True)
         return
 
     import torch

@@ -13,13 +13,29 @@
 namespace wotk {
 namespace {
 
-// column sums of T (I x J): one thread per column, rows in a fixed order
-__global__ __launch_bounds__(256) void colsum_kernel(const double *__restrict__ T, int64_t ld, int64_t I, int64_t J,
-                                                     double *__restrict__ colsum) {
+constexpr int kColRows = 128;  // rows per partial column sum
+
+// partial column sums of T (I x J) over blocks of kColRows rows (fixed order)
+__global__ __launch_bounds__(256) void colsum_partial_kernel(const double *__restrict__ T, int64_t ld, int64_t I,
+                                                             int64_t J, double *__restrict__ part) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.y * kColRows;
+    if (j >= J) return;
+    double s = 0.0;
+    for (int r = 0; r < kColRows; ++r) {
+        const int64_t i = r0 + r;
+        if (i >= I) break;
+        s += T[i * ld + j];
+    }
+    part[(int64_t)blockIdx.y * J + j] = s;
+}
+
+__global__ __launch_bounds__(256) void colsum_final_kernel(const double *__restrict__ part, int n_part, int64_t J,
+                                                           double *__restrict__ colsum) {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= J) return;
     double s = 0.0;
-    for (int64_t i = 0; i < I; ++i) s += T[i * ld + j];
+    for (int p = 0; p < n_part; ++p) s += part[(int64_t)p * J + j];
     colsum[j] = s;
 }
 
@@ -67,8 +83,13 @@ using namespace wotk;
 extern "C" int wotk_interp_weights(const double *T, int64_t ld, int64_t I, int64_t J, double frac, double *W,
                                    double *colsum, void *stream) {
     WOTK_REQUIRE(T && W && colsum && I > 0 && J > 0 && ld >= J, "interp_weights: bad arguments");
+    const int64_t n_part = (I + kColRows - 1) / kColRows;
+    WOTK_REQUIRE(n_part < 65536, "interp_weights: too many rows (%lld)", (long long)I);
     hipStream_t st = (hipStream_t)stream;
-    colsum_kernel<<<(unsigned)((J + 255) / 256), 256, 0, st>>>(T, ld, I, J, colsum);
+    // the partial sums live in W until the weights overwrite it (W holds I x J >= n_part x J doubles)
+    colsum_partial_kernel<<<dim3((unsigned)((J + 255) / 256), (unsigned)n_part), 256, 0, st>>>(T, ld, I, J, W);
+    WOTK_LAUNCH_CHECK();
+    colsum_final_kernel<<<(unsigned)((J + 255) / 256), 256, 0, st>>>(W, (int)n_part, J, colsum);
     WOTK_LAUNCH_CHECK();
     for (int64_t r0 = 0; r0 < I; r0 += 65535) {
         const int64_t rows = I - r0 < 65535 ? I - r0 : 65535;
