@@ -11,7 +11,8 @@
 // search, node potentials pi with reduced cost C_ij + pi_i - pi_j, and the
 // flow of every tree arc stored on its lower node (non-tree arcs carry 0).
 // Pricing: block search over the real arcs (block ~ sqrt(n m)), entering arc
-// = most negative reduced cost in the first block that has one.  Leaving arc:
+// = most negative reduced cost in the first block that has one (rows walked
+// without divisions, four independent argmin lanes).  Leaving arc:
 // the last blocking arc of the cycle in its orientation (strongly feasible
 // trees, no cycling).  A pivot re-hangs the subtree cut off by the leaving arc
 // below the entering arc, shifting its potentials by the entering arc's
@@ -102,27 +103,69 @@ struct Simplex {
         block = std::max<int64_t>(16, (int64_t)std::sqrt((double)(n * m)));
     }
 
-    // block search: the most negative reduced cost in the first block that has one
+    // block search: the most negative reduced cost in the first block that has
+    // one; arcs are visited row by row from next_arc on (cyclically), a block
+    // being `block` consecutive arcs
     int64_t entering() {
         const int64_t E = n * m;
-        int64_t best = -1, seen = 0, e = next_arc;
+        int64_t i = next_arc / m, j = next_arc % m;
+        int64_t seen = 0, in_block = 0, best = -1;
         double best_rc = -eps;
+        const double *pj = pi.data() + n;
         while (seen < E) {
-            const int64_t stop = std::min<int64_t>(E - seen, block);
-            for (int64_t k = 0; k < stop; ++k) {
-                const int64_t i = e / m, j = e % m;
-                const double rc = C[i * ldc + j] + pi[i] - pi[n + j];
-                if (rc < best_rc) {
-                    best_rc = rc;
-                    best = e;
+            const double *Crow = C + i * ldc;
+            const double pii = pi[i];
+            int64_t stop = std::min<int64_t>(m, j + (block - in_block));
+            stop = std::min<int64_t>(stop, j + (E - seen));
+            // four independent argmin lanes (first index wins ties, as a sequential scan)
+            double lrc[4] = {best_rc, best_rc, best_rc, best_rc};
+            int64_t lj[4] = {-1, -1, -1, -1};
+            int64_t jj = j;
+            for (; jj + 4 <= stop; jj += 4)
+                for (int l = 0; l < 4; ++l) {
+                    const double rc = Crow[jj + l] + pii - pj[jj + l];
+                    if (rc < lrc[l]) {
+                        lrc[l] = rc;
+                        lj[l] = jj + l;
+                    }
                 }
-                if (++e == E) e = 0;
+            for (; jj < stop; ++jj) {
+                const double rc = Crow[jj] + pii - pj[jj];
+                if (rc < lrc[0]) {
+                    lrc[0] = rc;
+                    lj[0] = jj;
+                }
             }
-            seen += stop;
-            if (best >= 0) {
-                next_arc = e;
-                return best;
+            int64_t bj = -1;
+            double brc = best_rc;
+            for (int l = 0; l < 4; ++l)
+                if (lj[l] >= 0 && (lrc[l] < brc || (lrc[l] == brc && bj >= 0 && lj[l] < bj))) {
+                    brc = lrc[l];
+                    bj = lj[l];
+                }
+            if (bj >= 0) {
+                best_rc = brc;
+                best = i * m + bj;
             }
+            const int64_t cnt = stop - j;
+            seen += cnt;
+            in_block += cnt;
+            j = stop;
+            if (j == m) {
+                j = 0;
+                if (++i == n) i = 0;
+            }
+            if (in_block >= block) {
+                if (best >= 0) {
+                    next_arc = i * m + j;
+                    return best;
+                }
+                in_block = 0;
+            }
+        }
+        if (best >= 0) {
+            next_arc = i * m + j;
+            return best;
         }
         return -1;
     }
