@@ -29,6 +29,8 @@ namespace {
 constexpr int kPopTile = 16;          // populations per MFMA tile
 constexpr int kWaves = 4;             // waves per workgroup (no LDS, no barriers)
 constexpr int kTargetWaves = 4096;    // ~4 waves per SIMD on 256 CUs
+constexpr int kPushBlocks = 4;        // 16-column MFMA blocks per push wave (4 or 8)
+constexpr int kPushCols = 16 * kPushBlocks;
 constexpr int kPushSteps = 4;         // MFMA k-steps (4 rows each) per push trip
 constexpr bool kPushPipelined = false; // push trips double-buffered (measured slower: fewer waves fit)
 constexpr int kPullRowBlocks = 4;     // 16-row MFMA blocks per pull wave
@@ -50,6 +52,10 @@ __device__ inline double pop_scale(const double *__restrict__ scale, int group, 
     return scale ? 1.0 / scale[group * kPopTile + pop] : 1.0;
 }
 
+template <typename T> struct Vec16;  // 16 bytes of map entries
+template <> struct Vec16<double> { typedef d2 type; };
+template <> struct Vec16<float> { typedef f4 type; };
+
 // Map entries are read once per walk: non-temporal loads keep them from
 // evicting the population vectors (re-read by every workgroup) from L2.
 template <typename V>
@@ -59,52 +65,44 @@ __device__ inline V stream_load(const V *p) {
 
 // ---------------------------------------------------------------------------
 // push: part[split][group][j][16] = sum_{i in split} T[i][j] * p[group][i][16]
-// One wave owns 64 columns of T.  FP64: the lane reads two double2 (columns
-// j0+2m.. and j0+32+2m..); FP32: one float4 (columns j0+4m..).  push_col
-// gives the column of MFMA block b at A-row m.  16 rows (4 MFMA k-steps) per
-// trip; the next trip's loads are issued before this trip's MFMAs.  FULL:
-// the wave's 64 columns are inside the map and rows are 16-byte aligned
-// (vector loads, no checks); otherwise element loads with column checks.
+// One wave owns kPushCols = 16 kPushBlocks columns of T (kPushBlocks MFMA
+// blocks).  The lane reads 16-byte vectors of its row: FP64 vector v holds
+// columns j0 + 32 v + 2m, +1; FP32 vector v columns j0 + 64 v + 4m .. +3.
+// push_col gives the column of MFMA block b at A-row m.  4 kPushSteps rows per
+// trip.  FULL: the wave's columns are inside the map and rows are 16-byte
+// aligned (vector loads, no checks); otherwise element loads with column
+// checks.
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ inline int64_t push_col(int64_t j0, int b, int m) {
-    if constexpr (sizeof(T) == 8) return j0 + 2 * m + (b & 1) + 32 * (b >> 1);
-    else return j0 + 4 * m + b;
+    if constexpr (sizeof(T) == 8) return j0 + 32 * (b >> 1) + 2 * m + (b & 1);
+    else return j0 + 64 * (b >> 2) + 4 * m + (b & 3);
 }
 
-// one lane's 4 entries of one map row, as loaded
-template <typename T> struct PushRow;
-template <> struct PushRow<double> {
-    d2 x, y;
-    __device__ double get(int b) const { return b < 2 ? x[b] : y[b - 2]; }
-};
-template <> struct PushRow<float> {
-    f4 x;
-    __device__ double get(int b) const { return (double)x[b]; }
+// one lane's kPushBlocks entries of one map row, as loaded
+template <typename T>
+struct PushRow {
+    static constexpr int PER = 16 / sizeof(T);
+    static constexpr int NV = kPushBlocks / PER;
+    typename Vec16<T>::type v[NV];
+    __device__ double get(int b) const { return (double)v[b / PER][b % PER]; }
 };
 
 template <typename T, bool FULL>
 __device__ inline void push_fetch(PushRow<T> &o, const T *__restrict__ row, int64_t j0, int m, int64_t n_cols) {
+    typedef typename Vec16<T>::type V;
+    constexpr int PER = PushRow<T>::PER;
+    constexpr int CW = 16 * PER;  // columns one vector load covers across the 16 lanes of a row
     if constexpr (FULL) {
-        if constexpr (sizeof(T) == 8) {
-            o.x = stream_load(reinterpret_cast<const d2 *>(row + j0 + 2 * m));
-            o.y = stream_load(reinterpret_cast<const d2 *>(row + j0 + 32 + 2 * m));
-        } else {
-            o.x = stream_load(reinterpret_cast<const f4 *>(row + j0 + 4 * m));
-        }
-    } else {
-        T v[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+        for (int v = 0; v < PushRow<T>::NV; ++v)
+            o.v[v] = stream_load(reinterpret_cast<const V *>(row + j0 + CW * v + PER * m));
+    } else {
+#pragma unroll
+        for (int b = 0; b < kPushBlocks; ++b) {
             const int64_t j = push_col<T>(j0, b, m);
             const T e = row[j < n_cols ? j : 0];
-            v[b] = j < n_cols ? e : T(0);
-        }
-        if constexpr (sizeof(T) == 8) {
-            o.x = d2{v[0], v[1]};
-            o.y = d2{v[2], v[3]};
-        } else {
-            o.x = f4{v[0], v[1], v[2], v[3]};
+            o.v[b / PER][b % PER] = j < n_cols ? e : T(0);
         }
     }
 }
@@ -127,19 +125,19 @@ __device__ inline void push_trip_fetch(PushTrip<T> &t, const T *__restrict__ map
 }
 
 template <typename T>
-__device__ inline void push_trip_mfma(const PushTrip<T> &t, double inv, d4 acc[4]) {
+__device__ inline void push_trip_mfma(const PushTrip<T> &t, double inv, d4 acc[kPushBlocks]) {
 #pragma unroll
     for (int u = 0; u < kPushSteps; ++u) {
         const double bv = t.b[u] * inv;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[b] = mfma_f64(t.a[u].get(b), bv, acc[b]);
+        for (int b = 0; b < kPushBlocks; ++b) acc[b] = mfma_f64(t.a[u].get(b), bv, acc[b]);
     }
 }
 
 template <typename T, bool FULL>
 __device__ inline void push_tile(const T *__restrict__ map, int64_t ld, int64_t i_begin, int64_t i_end,
                                  int64_t n_cols, const double *__restrict__ P, double inv, int64_t j0, int m, int q,
-                                 d4 acc[4]) {
+                                 d4 acc[kPushBlocks]) {
     // full trips of 4 kPushSteps rows; pipelined: trip t+1 loads while trip t multiplies
     constexpr int TR = 4 * kPushSteps;
     const int64_t n_trips = (i_end - i_begin) / TR;
@@ -171,7 +169,7 @@ __device__ inline void push_tile(const T *__restrict__ map, int64_t ld, int64_t 
         push_fetch<T, FULL>(a, map + rr * ld, j0, m, n_cols);
         const double bv = ok ? P[rr * kPopTile + m] * inv : 0.0;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[b] = mfma_f64(ok ? a.get(b) : 0.0, bv, acc[b]);
+        for (int b = 0; b < kPushBlocks; ++b) acc[b] = mfma_f64(ok ? a.get(b) : 0.0, bv, acc[b]);
     }
 }
 
@@ -186,23 +184,23 @@ __global__ __launch_bounds__(256) void push_partial_kernel(const T *__restrict__
     if (tile >= n_tiles) return;
     const int split = blockIdx.y, group = blockIdx.z, n_groups = gridDim.z;
     const int m = lane & 15, q = lane >> 4;
-    const int64_t j0 = (int64_t)tile * 64;
+    const int64_t j0 = (int64_t)tile * kPushCols;
     const int64_t i_begin = (int64_t)split * split_rows;
     const int64_t i_end = min(n_rows, i_begin + split_rows);
     const double *P = pops + (int64_t)group * n_rows * kPopTile;
     const double inv = pop_scale(scale, group, m);
 
-    d4 acc[4];
+    d4 acc[kPushBlocks];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[b] = d4{0.0, 0.0, 0.0, 0.0};
-    if (VEC && j0 + 64 <= n_cols)
+    for (int b = 0; b < kPushBlocks; ++b) acc[b] = d4{0.0, 0.0, 0.0, 0.0};
+    if (VEC && j0 + kPushCols <= n_cols)
         push_tile<T, VEC>(map, ld, i_begin, i_end, n_cols, P, inv, j0, m, q, acc);
     else
         push_tile<T, false>(map, ld, i_begin, i_end, n_cols, P, inv, j0, m, q, acc);
 
     double *out = part + ((int64_t)split * n_groups + group) * n_cols * kPopTile;
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int b = 0; b < kPushBlocks; ++b)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int64_t j = push_col<T>(j0, b, q + 4 * r);
@@ -219,10 +217,6 @@ __global__ __launch_bounds__(256) void push_partial_kernel(const T *__restrict__
 // j + E*q + s.  Full steps are double-buffered like the push trips; a partial
 // last step (element loads with column checks) follows them.
 // ---------------------------------------------------------------------------
-template <typename T> struct Vec16;  // 16 bytes of map entries
-template <> struct Vec16<double> { typedef d2 type; };
-template <> struct Vec16<float> { typedef f4 type; };
-
 // one lane's E = kPullVecs * 16 / sizeof(T) contiguous entries of one row
 template <typename T>
 struct PullRow {
@@ -516,7 +510,8 @@ StepShape step_shape(const wotk_tmap &m, int direction, int n_groups) {
     s.n_in = push ? m.rows : m.cols;
     s.n_out = push ? m.cols : m.rows;
     s.reduce_len = s.n_in;
-    const int64_t tiles = push ? (m.cols + 63) / 64 : (m.rows + 16 * kPullRowBlocks - 1) / (16 * kPullRowBlocks);
+    const int64_t tiles = push ? (m.cols + kPushCols - 1) / kPushCols
+                               : (m.rows + 16 * kPullRowBlocks - 1) / (16 * kPullRowBlocks);
     s.n_tiles = (int)tiles;
     const int64_t quantum = push ? 16 : pull_step_cols(m.dtype);
     // enough splits to put ~kTargetWaves waves in flight, each reduction chunk >= 8 quanta
