@@ -334,3 +334,20 @@ def test_otmodel_h5ad_course_through_from_directory(tm, tmp_path):
     assert rel(traj.X, O.trajectories(maps, P, 1)) < 1e-13
     f = tmm.fates(pops)
     assert rel(f.X, O.fates(maps, P, 1)) < 1e-13
+
+
+def test_model_get_coupling_chains_in_fp64(tm):
+    """Non-consecutive pairs chain the device maps in FP64 like np.dot
+    (chaining.py:4-40): equal to the oracle chain to rounding."""
+    from oracle import ot_oracle
+    rng = np.random.default_rng(51)
+    sizes = [90, 70, 110, 60]
+    maps, meta, tmaps, ids = _course(rng, sizes)
+    model = tm.TransportMapModel(tmaps, meta)
+    cp = model.get_coupling(0.0, 3.0)
+    want = ot_oracle.chain(maps)
+    assert cp.X.shape == (90, 60)
+    assert rel(cp.X, want) < 1e-14
+    assert list(cp.obs.index) == ids[0] and list(cp.var.index) == ids[3]
+    cp = model.get_coupling(1.0, 3.0)
+    assert rel(cp.X, ot_oracle.chain(maps[1:])) < 1e-14

@@ -189,8 +189,36 @@ class TransportMapModel:
             if src is None:
                 raise ValueError('No transport map found for {}', key)
             return self._as_anndata(src, key)
-        from . import chain_transport_maps
-        return chain_transport_maps(self, find_path(t0, t1, self.day_pairs, self.timepoints))
+        return self._chained_coupling(find_path(t0, t1, self.day_pairs, self.timepoints))
+
+    def _chained_coupling(self, path):
+        """T_0 T_1 ... over the device maps of ``path`` (wot/tmap/chaining.py:4-40
+        with glue_transport_maps, util.py:74-94): FP64 products (rocBLAS DGEMM),
+        rows of each next map re-ordered to the previous map's columns by cell
+        id as the reference does, no normalisation."""
+        torch = _dev.torch_mod()
+        for i in range(len(path) - 1):
+            if path[i][1] != path[i + 1][0]:
+                raise ValueError("Transport maps cannot be glued together : invalid path")
+        for a, b in path:
+            if a >= b:
+                raise ValueError("({}, {}) is not a valid transport map : it goes backwards in time".format(a, b))
+        first = self.device_map(*path[0])
+        X = first.X.double()
+        var = first.var_names
+        for a, b in path[1:]:
+            nxt = self.device_map(a, b)
+            Y = nxt.X.double()
+            if var is not None and nxt.obs_names is not None and not pd.Index(nxt.obs_names).equals(pd.Index(var)):
+                idx = pd.Index(nxt.obs_names).get_indexer_for(pd.Index(var))
+                Y = Y.index_select(0, torch.from_numpy(np.asarray(idx, dtype=np.int64)).to(Y.device))
+            X = X @ Y
+            var = nxt.var_names
+        n0, n1 = X.shape
+        obs = pd.DataFrame(index=first.obs_names if first.obs_names is not None else
+                           pd.RangeIndex(n0).astype(str))
+        var_df = pd.DataFrame(index=var if var is not None else pd.RangeIndex(n1).astype(str))
+        return _anndata.AnnData(X.cpu().numpy(), obs, var_df)
 
     def _as_anndata(self, src, key):
         if isinstance(src, DeviceMap):
