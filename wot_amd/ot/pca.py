@@ -9,8 +9,10 @@ scaled by ``singular_values_`` when the cost is built (ot_model.py:301, 245-247)
 
 Here the double centering is one libwotk kernel pass (``wotk_pca_center``).
 The top-k singular triplets come from the FP64 Gram matrix of the smaller side
-(one rocBLAS GEMM plus a symmetric eigensolver) when that side has at most
-``EXACT_MAX`` entries.  Otherwise they come from a seeded block subspace
+(one rocBLAS GEMM) when that side has at most ``EXACT_MAX`` entries: its top-k
+eigenpairs by Rayleigh-Ritz subspace iteration converged to residuals of
+``RITZ_TOL`` times the largest eigenvalue (the full symmetric eigensolver when
+that does not converge or k is not small).  Otherwise they come from a seeded block subspace
 iteration (GEMMs plus thin QR), with ``POWER_ITERS`` iterations and
 ``OVERSAMPLE`` extra columns.  Signs follow one fixed convention: each
 component's largest-magnitude cell coordinate is positive.  Squared distances
@@ -28,6 +30,9 @@ EXACT_MAX = 8192
 OVERSAMPLE = 10
 POWER_ITERS = 10
 SEED = 58951
+RITZ_TOL = 1e-13            # Gram eigenpairs: residual / largest eigenvalue
+RITZ_ABORT = 1e-6           # residual after 4 iterations above this: use the full solver
+MAX_SUBSPACE_ITERS = 12
 
 
 class DevicePCA:
@@ -89,20 +94,71 @@ def center(x):
     return A, mu, nu
 
 
+def _gram_top(torch, M, k):
+    """Top-k eigenpairs of the symmetric PSD Gram matrix M (device, FP64).
+
+    Block subspace iteration with l = min(2k + 10, size) columns: Y = M Q, an
+    orthonormal basis of span(Y) (symmetric orthonormalisation, twice), then
+    Rayleigh-Ritz with the small l x l eigenproblems solved on the host (they
+    are launch-bound on the device), until every top-k Ritz residual
+    ||M v - lambda v|| is below RITZ_TOL * lambda_max.  Falls back to the full
+    device eigensolver when l is not small next to the size, when the basis
+    degenerates, when the residual after 4 iterations is above RITZ_ABORT
+    (slow convergence: a flat spectrum past k), or after MAX_SUBSPACE_ITERS
+    iterations.  Returns (eigenvalues
+    descending, eigenvectors)."""
+    size = M.shape[0]
+    l = min(2 * k + 10, size)
+
+    def full():
+        w, Q = torch.linalg.eigh(M)
+        idx = torch.argsort(w, descending=True)[:k]
+        return w[idx], Q[:, idx]
+
+    if 4 * l > size:
+        return full()
+    dev = M.device
+    gen = torch.Generator(device=dev).manual_seed(SEED)
+    Q = torch.randn((size, l), dtype=torch.float64, device=dev, generator=gen)
+
+    def orthonormal(Y):
+        for _ in range(2):
+            G = (Y.T @ Y).cpu().numpy()
+            w, V = np.linalg.eigh(0.5 * (G + G.T))
+            if not w[0] > 1e-13 * w[-1]:
+                return None
+            Y = Y @ torch.from_numpy(V / np.sqrt(w)[None, :]).to(dev)
+        return Y
+
+    for it in range(MAX_SUBSPACE_ITERS):
+        Y = orthonormal(M @ Q)
+        if Y is None:
+            return full()
+        H = (Y.T @ (M @ Y)).cpu().numpy()
+        w, Z = np.linalg.eigh(0.5 * (H + H.T))
+        order = np.argsort(w)[::-1]
+        w, Z = w[order], Z[:, order]
+        Q = Y @ torch.from_numpy(np.ascontiguousarray(Z)).to(dev)
+        if it % 4 == 3:
+            wk = torch.from_numpy(w[:k].copy()).to(dev)
+            Vk = Q[:, :k]
+            rel_res = float(torch.linalg.vector_norm(M @ Vk - Vk * wk[None, :], dim=0).max()) / float(w[0])
+            if rel_res <= RITZ_TOL:
+                return wk, Vk
+            if it == 3 and rel_res > RITZ_ABORT:
+                break  # a flat spectrum past k: the full solver is faster than many more iterations
+    return full()
+
+
 def _top_exact(torch, A, k):
     n, g = A.shape
     if n <= g:
-        M = A @ A.T                      # cells Gram: eigenvectors are V
-        w, Q = torch.linalg.eigh(M)
-        idx = torch.argsort(w, descending=True)[:k]
-        S = torch.sqrt(torch.clamp(w[idx], min=0.0))
-        V = Q[:, idx]
+        w, V = _gram_top(torch, A @ A.T, k)   # cells Gram: eigenvectors are V
+        S = torch.sqrt(torch.clamp(w, min=0.0))
     else:
-        M = A.T @ A                      # genes Gram: eigenvectors are U, V = A U / S
-        w, Q = torch.linalg.eigh(M)
-        idx = torch.argsort(w, descending=True)[:k]
-        S = torch.sqrt(torch.clamp(w[idx], min=0.0))
-        V = (A @ Q[:, idx]) / torch.where(S > 0, S, torch.ones_like(S))[None, :]
+        w, U = _gram_top(torch, A.T @ A, k)   # genes Gram: eigenvectors are U, V = A U / S
+        S = torch.sqrt(torch.clamp(w, min=0.0))
+        V = (A @ U) / torch.where(S > 0, S, torch.ones_like(S))[None, :]
     return V, S
 
 
