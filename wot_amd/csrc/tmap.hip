@@ -107,55 +107,61 @@ __device__ inline void push_fetch(PushRow<T> &o, const T *__restrict__ row, int6
     }
 }
 
-template <typename T>
+// NGW population groups (16 populations each) per wave: one read of the map
+// feeds NGW x kPushBlocks MFMA accumulators
+template <typename T, int NGW>
 struct PushTrip {
     PushRow<T> a[kPushSteps];
-    double b[kPushSteps];
+    double b[NGW][kPushSteps];
 };
 
-template <typename T, bool FULL>
-__device__ inline void push_trip_fetch(PushTrip<T> &t, const T *__restrict__ map, int64_t ld, int64_t i,
-                                       const double *__restrict__ P, int64_t j0, int m, int q, int64_t n_cols) {
+template <typename T, bool FULL, int NGW>
+__device__ inline void push_trip_fetch(PushTrip<T, NGW> &t, const T *__restrict__ map, int64_t ld, int64_t i,
+                                       const double *const P[NGW], int64_t j0, int m, int q, int64_t n_cols) {
 #pragma unroll
     for (int u = 0; u < kPushSteps; ++u) {
         const int64_t r = i + 4 * u + q;
         push_fetch<T, FULL>(t.a[u], map + r * ld, j0, m, n_cols);
-        t.b[u] = P[r * kPopTile + m];
+#pragma unroll
+        for (int g = 0; g < NGW; ++g) t.b[g][u] = P[g][r * kPopTile + m];
     }
 }
 
-template <typename T>
-__device__ inline void push_trip_mfma(const PushTrip<T> &t, double inv, d4 acc[kPushBlocks]) {
+template <typename T, int NGW>
+__device__ inline void push_trip_mfma(const PushTrip<T, NGW> &t, const double inv[NGW], d4 acc[NGW][kPushBlocks]) {
 #pragma unroll
-    for (int u = 0; u < kPushSteps; ++u) {
-        const double bv = t.b[u] * inv;
+    for (int u = 0; u < kPushSteps; ++u)
 #pragma unroll
-        for (int b = 0; b < kPushBlocks; ++b) acc[b] = mfma_f64(t.a[u].get(b), bv, acc[b]);
-    }
+        for (int g = 0; g < NGW; ++g) {
+            const double bv = t.b[g][u] * inv[g];
+#pragma unroll
+            for (int b = 0; b < kPushBlocks; ++b) acc[g][b] = mfma_f64(t.a[u].get(b), bv, acc[g][b]);
+        }
 }
 
-template <typename T, bool FULL>
+template <typename T, bool FULL, int NGW>
 __device__ inline void push_tile(const T *__restrict__ map, int64_t ld, int64_t i_begin, int64_t i_end,
-                                 int64_t n_cols, const double *__restrict__ P, double inv, int64_t j0, int m, int q,
-                                 d4 acc[kPushBlocks]) {
+                                 int64_t n_cols, const double *const P[NGW], const double inv[NGW], int64_t j0, int m,
+                                 int q, d4 acc[NGW][kPushBlocks]) {
     // full trips of 4 kPushSteps rows; pipelined: trip t+1 loads while trip t multiplies
     constexpr int TR = 4 * kPushSteps;
     const int64_t n_trips = (i_end - i_begin) / TR;
-    PushTrip<T> A;
+    PushTrip<T, NGW> A;
     if constexpr (kPushPipelined) {
-        PushTrip<T> B;
-        if (n_trips > 0) push_trip_fetch<T, FULL>(A, map, ld, i_begin, P, j0, m, q, n_cols);
+        PushTrip<T, NGW> B;
+        if (n_trips > 0) push_trip_fetch<T, FULL, NGW>(A, map, ld, i_begin, P, j0, m, q, n_cols);
         int64_t t = 0;
         for (; t + 2 <= n_trips; t += 2) {
-            push_trip_fetch<T, FULL>(B, map, ld, i_begin + (t + 1) * TR, P, j0, m, q, n_cols);
+            push_trip_fetch<T, FULL, NGW>(B, map, ld, i_begin + (t + 1) * TR, P, j0, m, q, n_cols);
             push_trip_mfma(A, inv, acc);
-            if (t + 2 < n_trips) push_trip_fetch<T, FULL>(A, map, ld, i_begin + (t + 2) * TR, P, j0, m, q, n_cols);
+            if (t + 2 < n_trips)
+                push_trip_fetch<T, FULL, NGW>(A, map, ld, i_begin + (t + 2) * TR, P, j0, m, q, n_cols);
             push_trip_mfma(B, inv, acc);
         }
         if (t < n_trips) push_trip_mfma(A, inv, acc);
     } else {
         for (int64_t t = 0; t < n_trips; ++t) {
-            push_trip_fetch<T, FULL>(A, map, ld, i_begin + t * TR, P, j0, m, q, n_cols);
+            push_trip_fetch<T, FULL, NGW>(A, map, ld, i_begin + t * TR, P, j0, m, q, n_cols);
             push_trip_mfma(A, inv, acc);
         }
     }
@@ -167,45 +173,61 @@ __device__ inline void push_tile(const T *__restrict__ map, int64_t ld, int64_t 
         const int64_t rr = ok ? r : i_begin;
         PushRow<T> a;
         push_fetch<T, FULL>(a, map + rr * ld, j0, m, n_cols);
-        const double bv = ok ? P[rr * kPopTile + m] * inv : 0.0;
 #pragma unroll
-        for (int b = 0; b < kPushBlocks; ++b) acc[b] = mfma_f64(ok ? a.get(b) : 0.0, bv, acc[b]);
+        for (int g = 0; g < NGW; ++g) {
+            const double bv = ok ? P[g][rr * kPopTile + m] * inv[g] : 0.0;
+#pragma unroll
+            for (int b = 0; b < kPushBlocks; ++b) acc[g][b] = mfma_f64(ok ? a.get(b) : 0.0, bv, acc[g][b]);
+        }
     }
 }
 
-template <typename T, bool VEC>
+template <typename T, bool VEC, int NGW>
 __global__ __launch_bounds__(256) void push_partial_kernel(const T *__restrict__ map, int64_t ld, int64_t n_rows,
                                                            int64_t n_cols, const double *__restrict__ pops,
                                                            const double *__restrict__ scale,
                                                            double *__restrict__ part, int64_t split_rows,
-                                                           int n_tiles) {
+                                                           int n_tiles, int n_groups) {
     const int lane = threadIdx.x & 63;
     const int tile = blockIdx.x * kWaves + (threadIdx.x >> 6);
     if (tile >= n_tiles) return;
-    const int split = blockIdx.y, group = blockIdx.z, n_groups = gridDim.z;
+    const int split = blockIdx.y, g0 = blockIdx.z * NGW;
     const int m = lane & 15, q = lane >> 4;
     const int64_t j0 = (int64_t)tile * kPushCols;
     const int64_t i_begin = (int64_t)split * split_rows;
     const int64_t i_end = min(n_rows, i_begin + split_rows);
-    const double *P = pops + (int64_t)group * n_rows * kPopTile;
-    const double inv = pop_scale(scale, group, m);
-
-    d4 acc[kPushBlocks];
+    // a group past the last one multiplies group g0's values by 0 and is not written
+    const double *P[NGW];
+    double inv[NGW];
 #pragma unroll
-    for (int b = 0; b < kPushBlocks; ++b) acc[b] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int g = 0; g < NGW; ++g) {
+        const bool valid = g0 + g < n_groups;
+        P[g] = pops + (int64_t)(valid ? g0 + g : g0) * n_rows * kPopTile;
+        inv[g] = valid ? pop_scale(scale, g0 + g, m) : 0.0;
+    }
+
+    d4 acc[NGW][kPushBlocks];
+#pragma unroll
+    for (int g = 0; g < NGW; ++g)
+#pragma unroll
+        for (int b = 0; b < kPushBlocks; ++b) acc[g][b] = d4{0.0, 0.0, 0.0, 0.0};
     if (VEC && j0 + kPushCols <= n_cols)
-        push_tile<T, VEC>(map, ld, i_begin, i_end, n_cols, P, inv, j0, m, q, acc);
+        push_tile<T, VEC, NGW>(map, ld, i_begin, i_end, n_cols, P, inv, j0, m, q, acc);
     else
-        push_tile<T, false>(map, ld, i_begin, i_end, n_cols, P, inv, j0, m, q, acc);
+        push_tile<T, false, NGW>(map, ld, i_begin, i_end, n_cols, P, inv, j0, m, q, acc);
 
-    double *out = part + ((int64_t)split * n_groups + group) * n_cols * kPopTile;
 #pragma unroll
-    for (int b = 0; b < kPushBlocks; ++b)
+    for (int g = 0; g < NGW; ++g) {
+        if (g0 + g >= n_groups) break;
+        double *out = part + ((int64_t)split * n_groups + g0 + g) * n_cols * kPopTile;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t j = push_col<T>(j0, b, q + 4 * r);
-            if (j < n_cols) out[j * kPopTile + m] = acc[b][r];
-        }
+        for (int b = 0; b < kPushBlocks; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t j = push_col<T>(j0, b, q + 4 * r);
+                if (j < n_cols) out[j * kPopTile + m] = acc[g][b][r];
+            }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -583,12 +605,22 @@ void launch_partial(const wotk_tmap &m, int direction, const StepShape &sh, int 
     dim3 grid((sh.n_tiles + kWaves - 1) / kWaves, sh.n_split, n_groups);
     const bool vec = vec_ok<T>(m);
     if (direction == WOTK_PUSH_FORWARD) {
-        if (vec)
-            push_partial_kernel<T, true><<<grid, 256, 0, st>>>(map, m.ld, m.rows, m.cols, pops, scale, part,
-                                                              sh.split_len, sh.n_tiles);
-        else
-            push_partial_kernel<T, false><<<grid, 256, 0, st>>>(map, m.ld, m.rows, m.cols, pops, scale, part,
-                                                               sh.split_len, sh.n_tiles);
+        // two population groups per wave when there are several: one read of the map for both
+        if (n_groups > 1) {
+            grid.z = (n_groups + 1) / 2;
+            if (vec)
+                push_partial_kernel<T, true, 2><<<grid, 256, 0, st>>>(map, m.ld, m.rows, m.cols, pops, scale, part,
+                                                                     sh.split_len, sh.n_tiles, n_groups);
+            else
+                push_partial_kernel<T, false, 2><<<grid, 256, 0, st>>>(map, m.ld, m.rows, m.cols, pops, scale, part,
+                                                                      sh.split_len, sh.n_tiles, n_groups);
+        } else if (vec) {
+            push_partial_kernel<T, true, 1><<<grid, 256, 0, st>>>(map, m.ld, m.rows, m.cols, pops, scale, part,
+                                                                 sh.split_len, sh.n_tiles, n_groups);
+        } else {
+            push_partial_kernel<T, false, 1><<<grid, 256, 0, st>>>(map, m.ld, m.rows, m.cols, pops, scale, part,
+                                                                  sh.split_len, sh.n_tiles, n_groups);
+        }
     } else {
         if (vec)
             pull_partial_kernel<T, true><<<grid, 256, 0, st>>>(map, m.ld, m.rows, m.cols, pops, scale, part,
