@@ -51,3 +51,40 @@ def test_errors_map_to_reference_exceptions():
     with pytest.raises(ValueError):
         _native.check(rc)
     assert 'bad shape' in lib.wotk_last_error().decode()
+
+
+def test_new_entry_points_reject_bad_arguments_on_the_host():
+    """Argument checks of the map-walk, PCA, interpolation and EMD entry points
+    run before any device work (no GPU needed)."""
+    import ctypes
+    lib = _native.lib()
+    T = _native.TMap
+    # a chain whose second map does not take the first one's output cells
+    maps = (T * 2)(T(8, _native.DTYPE_F64, 0, 5, 4, 5, None, 0), T(8, _native.DTYPE_F64, 0, 3, 7, 3, None, 0))
+    assert lib.wotk_tmap_workspace_size(maps, 2, _native.PUSH_FORWARD, 3) == 0
+    outs = (ctypes.c_void_p * 2)(8, 8)
+    rc = lib.wotk_tmap_propagate(maps, 2, _native.PUSH_FORWARD, 8, 3, 3, 1, outs, 3, 8, 1 << 20, None, None)
+    assert rc == _native.WOTK_EINVAL and 'takes 7 cells' in lib.wotk_last_error().decode()
+    one = (T * 1)(T(8, 7, 0, 5, 4, 5, None, 0))  # unknown dtype
+    rc = lib.wotk_tmap_propagate(one, 1, _native.PUSH_FORWARD, 8, 3, 3, 1, outs, 3, 8, 1 << 20, None, None)
+    assert rc == _native.WOTK_EINVAL and 'dtype' in lib.wotk_last_error().decode()
+    one = (T * 1)(T(8, _native.DTYPE_F64, 0, 5, 4, 5, 8, 2))  # transposed copy with ld < rows
+    assert lib.wotk_tmap_propagate(one, 1, _native.PULL_BACK, 8, 3, 3, 1, outs, 3, 8, 1 << 20, None,
+                                   None) == _native.WOTK_EINVAL
+    good = (T * 1)(T(8, _native.DTYPE_F64, 0, 5, 4, 5, None, 0))
+    assert lib.wotk_tmap_workspace_size(good, 1, _native.PUSH_FORWARD, 20) > 0
+    assert lib.wotk_tmap_transpose(8, 3, 4, 2, 2, 8, 2, None) == _native.WOTK_EINVAL
+    assert lib.wotk_pca_center(8, 0, 2, 4, 3, None, 0, 8, 8, 8, 1 << 20, None) == _native.WOTK_EINVAL  # ldx < g
+    assert lib.wotk_pca_center_workspace_size(100, 50) > 0
+    assert lib.wotk_interp_weights(8, 2, 4, 3, 0.5, 8, 8, None) == _native.WOTK_EINVAL  # ld < J
+    assert lib.wotk_interp_gather(8, 1, 8, 1, 4, 8, 3, 3, 3, 0.5, 8, 3, None) == _native.WOTK_EINVAL  # ld0 < d
+    assert lib.wotk_rows_normalize_f64(8, 3, 4, 2, None) == _native.WOTK_EINVAL  # ld < k
+
+
+def test_emd_is_host_code():
+    """The exact EMD takes host pointers and needs no device."""
+    import numpy as np
+    from wot_amd.ot.emd import emd2
+    M = np.array([[0.0, 2.0], [1.0, 0.0]])
+    assert emd2(np.array([0.5, 0.5]), np.array([0.5, 0.5]), M) == 0.0
+    assert emd2(np.array([1.0, 0.0]), np.array([0.0, 1.0]), M) == 2.0
