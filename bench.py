@@ -681,6 +681,143 @@ def _tmap_cpu_baseline(counts, k, total_bytes, gpu_per):
             'GBps': rate / 1e9, 'gpu_over_cpu': (1.0 / gpu_per) / (rate / total_bytes)}
 
 
+GAP_SOLVER = dict(epsilon=0.05, lambda1=1.0, lambda2=50.0, epsilon0=1.0, tau=1e4, tolerance=1e-8, batch_size=5,
+                  max_iter=1e7)
+
+
+def cpu_gap_sample(x, y, growth, counts):
+    """FP64 numpy oracle of the duality-gap solver on the host cores, on a
+    bounded sample of the same map: cost + median at full size; one
+    exp(-C/eps) pass; two early-returning solves (max_iter 10 and 30) whose
+    difference is 20 iterations; one plan build + primal + dual.  Extrapolated
+    with the device run's counts: 2 kernel builds per epsilon scaling plus one
+    per stabilisation, its iterations, and one primal / dual check per batch
+    of the last scaling (the earlier scalings' checks are O(I + J))."""
+    from oracle import ot_oracle as O
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get('num_threads', 1) for i in threadpool_info()] + [1])
+    except Exception:  # pragma: no cover
+        threads = os.cpu_count()
+    t0 = time.perf_counter()
+    C = O.default_cost_matrix(x, y)
+    t_cost = time.perf_counter() - t0
+    eps = GAP_SOLVER['epsilon']
+    np.exp(-C[:2] / eps)  # first-call overhead out of the timed pass
+    t0 = time.perf_counter()
+    Kt = np.exp(-C / eps)
+    t_exp = time.perf_counter() - t0
+    with np.errstate(all='ignore'):
+        t0 = time.perf_counter()
+        O.optimal_transport_duality_gap(C, growth, **dict(GAP_SOLVER, max_iter=10))
+        t_a = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.optimal_transport_duality_gap(C, growth, **dict(GAP_SOLVER, max_iter=30))
+        t_b = time.perf_counter() - t0
+        n_i, n_j = C.shape
+        dx, dy = np.ones(n_i) / n_i, np.ones(n_j) / n_j
+        p, q = growth, np.ones(n_j) * np.average(growth)
+        a, b = np.ones(n_i), np.ones(n_j)
+        t0 = time.perf_counter()
+        R = (Kt.T * a).T * b
+        O.primal(C, Kt, R, dx, dy, p, q, a, b, eps, GAP_SOLVER['lambda1'], GAP_SOLVER['lambda2'])
+        O.dual(C, Kt, R, dx, dy, p, q, a, b, eps, GAP_SOLVER['lambda1'], GAP_SOLVER['lambda2'])
+        t_check = time.perf_counter() - t0
+    del C, Kt, R
+    t_it = max((t_b - t_a) / 20.0, 0.0)
+    n_build = 2 * 6 + counts['stabilisations']
+    per_map = t_cost + n_build * t_exp + counts['iterations'] * t_it + counts['final_checks'] * t_check
+    return per_map, threads, dict(t_cost_s=t_cost, t_exp_s=t_exp, t_iter_s=t_it, t_check_s=t_check,
+                                  sample_wall_s=t_cost + t_exp + t_a + t_b + t_check)
+
+
+def bench_gap(args):
+    """One 15k x 15k map with OTModel's default solver (duality gap,
+    tolerance 1e-8, growth_iters=1): cost + median + solve + FP64 plan, inputs
+    resident in HBM; plain-GPU restatement and CPU oracle beside it."""
+    import torch
+    import wot_amd.ot as ot  # noqa: F401  (loads the library)
+    from wot_amd.ot import device as dev
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(CFG['n_source'], CFG['n_target'], dim=CFG['dim'], n_components=CFG['n_components'],
+                  seed=CFG['seed'])
+    x = torch.from_numpy(dp.x).cuda()
+    y = torch.from_numpy(dp.y).cuda()
+    g = torch.from_numpy(dp.growth).cuda()
+    n_i, n_j = CFG['n_source'], CFG['n_target']
+
+    def one():
+        cost = dev.DeviceCost.from_coordinates(x, y, mode=args.cost_mode)
+        return cost, dev.solve(cost, g, 'duality_gap', GAP_SOLVER, want_plan=True)
+
+    for w in range(args.warmup):
+        log('warmup %d' % w)
+        one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        cost, res = one()
+        log('step %d done' % s)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = res['stats']
+    # the early scalings do not depend on the tolerance: the same solve stopped
+    # after one batch of the last scaling gives the last scaling's share
+    early = dev.solve(cost, g, 'duality_gap', dict(GAP_SOLVER, tolerance=1e300), want_plan=False)['stats']
+    final_iters = st['iterations'] - (early['iterations'] - GAP_SOLVER['batch_size'])
+    counts = dict(iterations=st['iterations'], stabilisations=st['stabilisations'], gap_checks=st['gap_checks'],
+                  final_iterations=final_iters, final_checks=final_iters // GAP_SOLVER['batch_size'],
+                  gap=st['gap'])
+    maps_per_s = args.steps / elapsed
+    sweep_bytes = 2.0 * st['iterations'] * 4.0 * n_i * n_j
+    out = {
+        'metric': 'transport maps/sec, duality-gap solver (OTModel default)', 'value': maps_per_s,
+        'unit': 'maps/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': 1000.0 * elapsed / args.steps, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64/f32', 'data': 'synthetic (seeded Gaussian mixture in 30-d PCA space)',
+        'config': {'workload': 'one day pair 15000x15000, d=30, 10 components, duality_gap, growth_iters=1',
+                   **GAP_SOLVER, 'cost_mode': args.cost_mode},
+        'solver_counts': counts,
+        'sinkhorn_iters_per_s': maps_per_s * st['iterations'],
+        'sweep_bytes_per_s_end_to_end': sweep_bytes * maps_per_s / 1e9,
+    }
+    if not args.no_baselines:
+        from wot_amd import plain_gpu
+        log('plain GPU sample')
+        plan = res['plan']
+        del res
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        c64 = dev.sqeuclid_f64(x, y)
+        v = torch.sort(c64.view(-1)).values
+        med = (v[v.numel() // 2 - 1] + v[v.numel() // 2]) / 2
+        C = c64 / med
+        del v, c64
+        torch.cuda.synchronize()
+        t_cost = time.perf_counter() - t0
+        pst = {}
+        t0 = time.perf_counter()
+        plain = plain_gpu.optimal_transport_duality_gap(C, g, stats=pst, **GAP_SOLVER)
+        torch.cuda.synchronize()
+        t_solve = time.perf_counter() - t0
+        rel = float(torch.linalg.norm(plan - plain) / torch.linalg.norm(plain))
+        del plain, C, plan
+        torch.cuda.empty_cache()
+        out['plain_gpu'] = {'value': 1.0 / (t_cost + t_solve), 'unit': 'maps/s', 't_cost_s': t_cost,
+                            't_solve_s': t_solve, 'counts': pst,
+                            'sample': 'torch FP64 dense-K restatement of the duality-gap solver, one full map'}
+        out['speedup_vs_plain_gpu'] = maps_per_s * (t_cost + t_solve)
+        out['parity_vs_plain_gpu_rel_frob'] = rel
+        log('cpu sample')
+        per_map_cpu, threads, detail = cpu_gap_sample(dp.x, dp.y, dp.growth, counts)
+        out['cpu_baseline'] = {'value': 1.0 / per_map_cpu, 'unit': 'maps/s', 'cores': threads, 'kind': 'port',
+                               'sample': 'FP64 numpy oracle at full size: cost+median, one kernel build, 20 '
+                                         'iterations (difference of two early-returning solves), one primal/dual '
+                                         'check; extrapolated with the device run\'s counts', **detail}
+        out['speedup_vs_cpu'] = maps_per_s * per_map_cpu
+    return out
+
+
 PCA_CFG = dict(n_cells=(15000, 15000), genes=2000, rank=40, k=30, seed=11)
 
 
@@ -853,8 +990,9 @@ def main():
     ap.add_argument('--large-n', type=int, default=60000, help='large only: cells per day')
     ap.add_argument('--group-size', type=int, default=None,
                     help='timecourse only: combinations per batched run (default: the whole grid)')
-    ap.add_argument('--config', default='pair', choices=['pair', 'timecourse', 'adjoint', 'large', 'tmap', 'pca', 'validation'],
-                    help='pair: one 15k x 15k map (default); timecourse: 38 pairs x 9-point grid; '
+    ap.add_argument('--config', default='pair', choices=['pair', 'gap', 'timecourse', 'adjoint', 'large', 'tmap', 'pca', 'validation'],
+                    help='pair: one 15k x 15k map (default); gap: the same pair with the duality-gap solver; '
+                         'timecourse: 38 pairs x 9-point grid; '
                          'adjoint: gradient of the converged objective, 8k x 8k; '
                          'tmap: trajectories through the 38 maps of the time course; '
                          'pca: local PCA of one 15k + 15k cell day pair; '
@@ -862,7 +1000,7 @@ def main():
     ap.add_argument('--tmap-dtype', default='float64', choices=['float64', 'float32'],
                     help='tmap only: precision of the stored maps')
     ap.add_argument('--cost-mode', default='bf16x6', choices=['f64', 'bf16x3', 'bf16x6'],
-                    help='pair only: cost kernel of the timed maps')
+                    help='pair / gap: cost kernel of the timed maps')
     ap.add_argument('--no-tune', action='store_true',
                     help='skip the tile autotuner (use persisted or default tiles)')
     ap.add_argument('--iters', type=int, default=None,
@@ -872,6 +1010,9 @@ def main():
         print('bench.py measures one GPU: this build has no multi-GPU path (got --gpus %d)' % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.config == 'gap':
+        print(json.dumps(bench_gap(args)), flush=True)
+        return
     if args.config == 'timecourse':
         print(json.dumps(bench_timecourse(args)), flush=True)
         return

@@ -173,7 +173,10 @@ struct PlanArgs {
     int want_ktilde;   // include sum exp(-A/eps) in partials[2]
 };
 
-template <int ROWS>
+// PLAN / PARTS: compiled with or without the plan store and the gap partials,
+// so the gap pass (no plan) and the final plan (no partials) carry no
+// per-entry branch for the other
+template <int ROWS, bool PLAN, bool PARTS>
 __global__ __launch_bounds__(kThreads) void plan_rows_kernel(PlanArgs pa) {
     const int r0 = blockIdx.x * ROWS;
     const int tid = threadIdx.x;
@@ -201,9 +204,9 @@ __global__ __launch_bounds__(kThreads) void plan_rows_kernel(PlanArgs pa) {
                 const double y = (fr[r] + gv[q] - a) * pa.inv_eps;
                 const double R = exp(y);
                 acc[r] += R;
-                if (pa.P && r0 + r < pa.nrows && c + q < pa.ncols)
+                if (PLAN && r0 + r < pa.nrows && c + q < pa.ncols)
                     pa.P[(int64_t)(r0 + r) * pa.ldp + c + q] = R / pa.div;
-                if (pa.partials) {
+                if (PARTS) {
                     if (R > 0.0) s_ry = fma(R, y, s_ry);
                     s_rc = fma(R, a, s_rc);
                     if (pa.want_ktilde && r0 + r < pa.nrows && c + q < pa.ncols)
@@ -219,7 +222,7 @@ __global__ __launch_bounds__(kThreads) void plan_rows_kernel(PlanArgs pa) {
         const double v = wave_sum(acc[r]);
         if (lane == 0) red[r][wave] = v;
     }
-    if (pa.partials) {
+    if (PARTS) {
         const double a0 = wave_sum(s_ry), a1 = wave_sum(s_rc), a2 = wave_sum(s_kt);
         if (lane == 0) {
             red[ROWS][wave] = a0;
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(kThreads) void plan_rows_kernel(PlanArgs pa) {
         for (int w = 0; w < kThreads / 64; ++w) v += red[tid][w];
         if (tid < ROWS) {
             if (pa.row_sums && r0 + tid < pa.nrows) pa.row_sums[r0 + tid] = v / pa.div;
-        } else if (pa.partials) {
+        } else if (PARTS) {
             pa.partials[(int64_t)blockIdx.x * 3 + (tid - ROWS)] = v;
         }
     }
@@ -282,16 +285,17 @@ __global__ __launch_bounds__(1024) void stabilise_kernel(const double *__restric
                                                          long long *__restrict__ counter) {
     __shared__ double red[16];
     __shared__ int fire;
+    // max over the exponents, then one exp (exp is increasing)
     double m = -INFINITY;
-    for (int i = threadIdx.x; i < n_i; i += blockDim.x) m = fmax(m, exp((f[i] - u[i]) * inv_eps));
-    for (int j = threadIdx.x; j < n_j; j += blockDim.x) m = fmax(m, exp((g[j] - v[j]) * inv_eps));
+    for (int i = threadIdx.x; i < n_i; i += blockDim.x) m = fmax(m, (f[i] - u[i]) * inv_eps);
+    for (int j = threadIdx.x; j < n_j; j += blockDim.x) m = fmax(m, (g[j] - v[j]) * inv_eps);
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = red[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) t = fmax(t, red[w]);
-        fire = t > tau;
+        fire = exp(t) > tau;
         if (fire) counter[0] += 1;
     }
     __syncthreads();
@@ -373,10 +377,15 @@ struct GapArgs {
 };
 
 __global__ __launch_bounds__(1024) void gap_finish_kernel(GapArgs ga) {
-    __shared__ double red[6][16];
+    __shared__ double red[7][16];
     const double q = ga.misc[1];
     const double inv_i = 1.0 / ga.n_i, inv_j = 1.0 / ga.n_j;
-    double f1 = 0.0, f2 = 0.0, s1 = 0.0, s2 = 0.0, sr = 0.0;
+    double f1 = 0.0, f2 = 0.0, s1 = 0.0, s2 = 0.0, sr = 0.0, p_ry = 0.0, p_rc = 0.0;
+    // block partials: thread t takes blocks t, t + blockDim, ... (fixed order: deterministic)
+    for (int b = threadIdx.x; b < ga.nparts; b += blockDim.x) {
+        p_ry += ga.partials[3 * b];
+        p_rc += ga.partials[3 * b + 1];
+    }
     for (int i = threadIdx.x; i < ga.n_i; i += blockDim.x) {
         const double p = ga.G[i];
         const double x = ga.r[i] * inv_j;  // R.dot(dy)
@@ -394,6 +403,8 @@ __global__ __launch_bounds__(1024) void gap_finish_kernel(GapArgs ga) {
     s1 = wave_sum(s1);
     s2 = wave_sum(s2);
     sr = wave_sum(sr);
+    p_ry = wave_sum(p_ry);
+    p_rc = wave_sum(p_rc);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         red[0][w] = f1;
@@ -401,17 +412,15 @@ __global__ __launch_bounds__(1024) void gap_finish_kernel(GapArgs ga) {
         red[2][w] = s1;
         red[3][w] = s2;
         red[4][w] = sr;
+        red[5][w] = p_ry;
+        red[6][w] = p_rc;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double t[5] = {0, 0, 0, 0, 0};
-        for (int k = 0; k < 5; ++k)
+        double t[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 7; ++k)
             for (int x = 0; x < (int)(blockDim.x >> 6); ++x) t[k] += red[k][x];
-        double s_ry = 0.0, s_rc = 0.0;
-        for (int b = 0; b < ga.nparts; ++b) {  // fixed order: deterministic
-            s_ry += ga.partials[3 * b];
-            s_rc += ga.partials[3 * b + 1];
-        }
+        const double s_ry = t[5], s_rc = t[6];
         const double skt = ga.sum_ktilde_dev ? *ga.sum_ktilde_dev : ga.sum_ktilde;
         const double ij = (double)ga.n_i * (double)ga.n_j;
         const double pri = ga.lambda1 * t[0] + ga.lambda2 * t[1] +
@@ -424,12 +433,19 @@ __global__ __launch_bounds__(1024) void gap_finish_kernel(GapArgs ga) {
 }
 
 // sum of the K-tilde partials (slot 2 of each block) -> out
-__global__ void sum_slot_kernel(const double *__restrict__ partials, int nparts, int slot,
-                                double *__restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double t = 0.0;
-        for (int b = 0; b < nparts; ++b) t += partials[3 * b + slot];
-        *out = t;
+// (one block of 1024 threads; thread t takes blocks t, t + 1024, ...: fixed order)
+__global__ __launch_bounds__(1024) void sum_slot_kernel(const double *__restrict__ partials, int nparts,
+                                                        int slot, double *__restrict__ out) {
+    __shared__ double red[16];
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) t += partials[3 * b + slot];
+    t = wave_sum(t);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0.0;
+        for (int x = 0; x < (int)(blockDim.x >> 6); ++x) v += red[x];
+        *out = v;
     }
 }
 
@@ -537,7 +553,10 @@ static int launch_plan(const float *A, int64_t lda, int nrows, int ncols, const 
                        double *row_sums, double *partials, int want_kt, hipStream_t s) {
     PlanArgs pa{A, lda, nrows, ncols, f, g, 1.0 / eps, div, P, ldp, row_sums, partials, want_kt};
     const int nb = (nrows + kRows - 1) / kRows;
-    plan_rows_kernel<kRows><<<nb, kThreads, 0, s>>>(pa);
+    if (P && partials) plan_rows_kernel<kRows, true, true><<<nb, kThreads, 0, s>>>(pa);
+    else if (P) plan_rows_kernel<kRows, true, false><<<nb, kThreads, 0, s>>>(pa);
+    else if (partials) plan_rows_kernel<kRows, false, true><<<nb, kThreads, 0, s>>>(pa);
+    else plan_rows_kernel<kRows, false, false><<<nb, kThreads, 0, s>>>(pa);
     WOTK_LAUNCH_CHECK();
     return WOTK_OK;
 }
@@ -570,7 +589,7 @@ static int ktilde_sum(const Problem &pb, Work &w, double eps, double *out, hipSt
     int rc = launch_plan(pb.C, pb.ldc, pb.n_i, pb.n_j, w.f, w.g, eps, 1.0, nullptr, 0, nullptr,
                          w.partials, 1, s);
     if (rc) return rc;
-    sum_slot_kernel<<<1, 64, 0, s>>>(w.partials, (pb.n_i + kRows - 1) / kRows, 2, out);
+    sum_slot_kernel<<<1, 1024, 0, s>>>(w.partials, (pb.n_i + kRows - 1) / kRows, 2, out);
     WOTK_LAUNCH_CHECK();
     return WOTK_OK;
 }
