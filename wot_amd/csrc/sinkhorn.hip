@@ -173,10 +173,10 @@ struct PlanArgs {
     int want_ktilde;   // include sum exp(-A/eps) in partials[2]
 };
 
-// PLAN / PARTS: compiled with or without the plan store and the gap partials,
-// so the gap pass (no plan) and the final plan (no partials) carry no
-// per-entry branch for the other
-template <int ROWS, bool PLAN, bool PARTS>
+// PLAN / PARTS / KT: compiled with or without the plan store, the gap
+// partials and the K-tilde sum, so no pass carries a per-entry branch for
+// another's output
+template <int ROWS, bool PLAN, bool PARTS, bool KT>
 __global__ __launch_bounds__(kThreads) void plan_rows_kernel(PlanArgs pa) {
     const int r0 = blockIdx.x * ROWS;
     const int tid = threadIdx.x;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(kThreads) void plan_rows_kernel(PlanArgs pa) {
                 if (PARTS) {
                     if (R > 0.0) s_ry = fma(R, y, s_ry);
                     s_rc = fma(R, a, s_rc);
-                    if (pa.want_ktilde && r0 + r < pa.nrows && c + q < pa.ncols)
+                    if (KT && r0 + r < pa.nrows && c + q < pa.ncols)
                         s_kt += exp(-a * pa.inv_eps);
                 }
             }
@@ -553,10 +553,12 @@ static int launch_plan(const float *A, int64_t lda, int nrows, int ncols, const 
                        double *row_sums, double *partials, int want_kt, hipStream_t s) {
     PlanArgs pa{A, lda, nrows, ncols, f, g, 1.0 / eps, div, P, ldp, row_sums, partials, want_kt};
     const int nb = (nrows + kRows - 1) / kRows;
-    if (P && partials) plan_rows_kernel<kRows, true, true><<<nb, kThreads, 0, s>>>(pa);
-    else if (P) plan_rows_kernel<kRows, true, false><<<nb, kThreads, 0, s>>>(pa);
-    else if (partials) plan_rows_kernel<kRows, false, true><<<nb, kThreads, 0, s>>>(pa);
-    else plan_rows_kernel<kRows, false, false><<<nb, kThreads, 0, s>>>(pa);
+    if (P && partials && want_kt) plan_rows_kernel<kRows, true, true, true><<<nb, kThreads, 0, s>>>(pa);
+    else if (P && partials) plan_rows_kernel<kRows, true, true, false><<<nb, kThreads, 0, s>>>(pa);
+    else if (P) plan_rows_kernel<kRows, true, false, false><<<nb, kThreads, 0, s>>>(pa);
+    else if (partials && want_kt) plan_rows_kernel<kRows, false, true, true><<<nb, kThreads, 0, s>>>(pa);
+    else if (partials) plan_rows_kernel<kRows, false, true, false><<<nb, kThreads, 0, s>>>(pa);
+    else plan_rows_kernel<kRows, false, false, false><<<nb, kThreads, 0, s>>>(pa);
     WOTK_LAUNCH_CHECK();
     return WOTK_OK;
 }
