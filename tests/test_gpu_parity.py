@@ -82,6 +82,60 @@ def test_duality_gap_max_iter_early_return(ot, caplog):
     assert_parity(R, z['R'])
 
 
+GAP_DEFAULTS = dict(lambda1=1.0, lambda2=50.0, epsilon=0.05, batch_size=5, tolerance=1e-8, tau=1e4, epsilon0=1.0,
+                    max_iter=1e7)
+
+
+def _gap_problem(n_i=160, n_j=200, seed=5):
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(n_i, n_j, dim=10, n_components=4, seed=seed)
+    return O.default_cost_matrix(dp.x, dp.y), dp.growth
+
+
+@pytest.mark.parametrize("tau,batch", [(1e4, 5), (1.5, 5), (1e4, 3)])
+def test_duality_gap_counts_match_oracle(ot, tau, batch):
+    """The last epsilon scaling runs each primal/dual pass beside the next
+    batch and rolls that batch back once the gap has converged: the iteration
+    and stabilisation counts equal those of the oracle's sequential loop."""
+    C, G = _gap_problem()
+    params = dict(GAP_DEFAULTS, tau=tau, batch_size=batch)
+    print('case tau', tau, 'batch', batch, flush=True)
+    want, got = {}, {}
+    with np.errstate(all='ignore'):
+        R0 = O.optimal_transport_duality_gap(C, G, stats=want, **params)
+    R = ot.optimal_transport_duality_gap(C, G, stats=got, **params)
+    assert got['iterations'] == want['iterations']
+    assert got['stabilisations'] == want['stabilisations']
+    if tau < 100:
+        assert want['stabilisations'] > 0
+    assert_parity(R, R0)
+
+
+def test_duality_gap_max_iter_around_the_last_scaling(ot):
+    """max_iter landing before, on and after the batch boundaries of the last
+    scaling (where batches run ahead of their gap check): the same early,
+    un-normalised return as the oracle, or the converged plan beyond it."""
+    C, G = _gap_problem()
+    want = {}
+    with np.errstate(all='ignore'):
+        O.optimal_transport_duality_gap(C, G, stats=want, **GAP_DEFAULTS)
+        early = {}
+        O.optimal_transport_duality_gap(C, G, stats=early, **dict(GAP_DEFAULTS, tolerance=1e300))
+    n, b = want['iterations'], GAP_DEFAULTS['batch_size']
+    first = early['iterations'] - b  # iterations before the last scaling
+    assert n > first + 3 * b
+    for m in sorted({first + 1, first + b, first + b + 1, n - b - 1, n - b, n - 1, n, n + 1, n + b + 1}):
+        print('case max_iter', m, 'of', n, flush=True)
+        params = dict(GAP_DEFAULTS, max_iter=m)
+        w, g = {}, {}
+        with np.errstate(all='ignore'):
+            R0 = O.optimal_transport_duality_gap(C, G, stats=w, **params)
+        R = ot.optimal_transport_duality_gap(C, G, stats=g, **params)
+        assert g['iterations'] == w['iterations']
+        assert bool(g['max_iter_reached']) == bool(w['max_iter_reached']) == (m <= n)
+        assert_parity(R, R0)
+
+
 def test_reference_identity_case(ot):
     """tests/test_transport.py:11-34: integer diagonal cost -> identity plan."""
     C = np.array([[0, 100, 100], [100, 0, 100], [100, 100, 0]])

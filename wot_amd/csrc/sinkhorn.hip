@@ -309,6 +309,38 @@ __global__ void zero_ll_kernel(long long *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
+// state of the duality-gap solver's last scaling (f, g, u, v, stabilisation
+// count) to its snapshot (back = 0) or from it (back = 1: rolls back a batch
+// that ran ahead of its gap check)
+__global__ void snapshot_kernel(double *__restrict__ f, double *__restrict__ g, double *__restrict__ u,
+                                double *__restrict__ v, double *__restrict__ fs, double *__restrict__ gs,
+                                double *__restrict__ us, double *__restrict__ vs, int n_i, int n_j,
+                                long long *__restrict__ counter, int back) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_i; i += stride) {
+        if (back) {
+            f[i] = fs[i];
+            u[i] = us[i];
+        } else {
+            fs[i] = f[i];
+            us[i] = u[i];
+        }
+    }
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n_j; j += stride) {
+        if (back) {
+            g[j] = gs[j];
+            v[j] = vs[j];
+        } else {
+            gs[j] = g[j];
+            vs[j] = v[j];
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (back) counter[0] = counter[1];
+        else counter[1] = counter[0];
+    }
+}
+
 __global__ void copy2_kernel(double *__restrict__ d0, const double *__restrict__ s0, int n0,
                              double *__restrict__ d1, const double *__restrict__ s1, int n1) {
     const int stride = gridDim.x * blockDim.x;
@@ -463,6 +495,7 @@ struct Problem {
 
 struct Work {
     double *f, *g, *logp, *misc, *r, *c, *u, *v, *fp, *gp, *up, *vp, *partials, *gapout;
+    double *fs, *gs, *us, *vs;  // snapshot of the last scaling's state (gap pass ahead of the next batch)
     long long *counter;
     int nparts;
 };
@@ -484,6 +517,10 @@ static size_t carve(Arena &ar, int64_t I, int64_t J, Work *w) {
     w->nparts = (int)((std::max(I, J) + kRows - 1) / kRows);
     w->partials = ar.take<double>((size_t)3 * w->nparts);
     w->gapout = ar.take<double>(8);
+    w->fs = ar.take<double>(I4);
+    w->gs = ar.take<double>(J4);
+    w->us = ar.take<double>(I4);
+    w->vs = ar.take<double>(J4);
     w->counter = ar.take<long long>(4);
     return ar.off;
 }
@@ -569,16 +606,14 @@ int launch_plan_sums(const float *A, int64_t lda, int nrows, int ncols, const do
     return launch_plan(A, lda, nrows, ncols, f, g, eps, 1.0, nullptr, 0, row_sums, nullptr, 0, s);
 }
 
-// primal/dual/gap of the current state at eps -> w.gapout[0..2] (device)
-static int gap_pass(const Problem &pb, Work &w, double lambda1, double lambda2, double eps,
-                    const double *sum_kt_dev, hipStream_t s) {
-    int rc = launch_plan(pb.C, pb.ldc, pb.n_i, pb.n_j, w.f, w.g, eps, 1.0, nullptr, 0, w.r,
-                         w.partials, 0, s);
+// primal/dual/gap at potentials f, g and eps -> w.gapout[0..2] (device)
+static int gap_pass(const Problem &pb, Work &w, const double *f, const double *g, double lambda1,
+                    double lambda2, double eps, const double *sum_kt_dev, hipStream_t s) {
+    int rc = launch_plan(pb.C, pb.ldc, pb.n_i, pb.n_j, f, g, eps, 1.0, nullptr, 0, w.r, w.partials, 0, s);
     if (rc) return rc;
-    rc = launch_plan(pb.CT, pb.ldct, pb.n_j, pb.n_i, w.g, w.f, eps, 1.0, nullptr, 0, w.c, nullptr, 0,
-                     s);
+    rc = launch_plan(pb.CT, pb.ldct, pb.n_j, pb.n_i, g, f, eps, 1.0, nullptr, 0, w.c, nullptr, 0, s);
     if (rc) return rc;
-    GapArgs ga{w.r, w.c, w.f, w.g, pb.G, w.misc, w.partials, (pb.n_i + kRows - 1) / kRows, pb.n_i,
+    GapArgs ga{w.r, w.c, f, g, pb.G, w.misc, w.partials, (pb.n_i + kRows - 1) / kRows, pb.n_i,
                pb.n_j, lambda1, lambda2, eps, 0.0, sum_kt_dev, w.gapout};
     gap_finish_kernel<<<1, 1024, 0, s>>>(ga);
     WOTK_LAUNCH_CHECK();
@@ -685,6 +720,20 @@ extern "C" int wotk_transport_stablev2(const float *C, int64_t ldc, const float 
     return WOTK_OK;
 }
 
+// second stream and event of the duality-gap solver's last scaling; drained
+// and released on every exit path
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t snap = nullptr;
+    ~SideStream() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+        if (snap) (void)hipEventDestroy(snap);
+    }
+};
+
 extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const float *CT,
                                           int64_t ldct, int64_t I, int64_t J, const double *G,
                                           double lambda1, double lambda2, double epsilon,
@@ -707,59 +756,59 @@ extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const flo
     double gap = INFINITY;
     double host_out[3] = {0, 0, 0};
     double *sum_kt = w.misc + 4;
+    double inv_eps = 1.0 / eps_i;
+    // one (a, b) iteration plus the tau check (optimal_transport.py:131-141)
+    auto one_iteration = [&](double eps) -> int {
+        int r = iterate(pb, w, lambda1, lambda2, eps, s);
+        if (r) return r;
+        stabilise_kernel<<<1, 1024, 0, s>>>(w.f, w.u, pb.n_i, w.g, w.v, pb.n_j, inv_eps, tau, w.counter);
+        WOTK_LAUNCH_CHECK();
+        return WOTK_OK;
+    };
+    // early, un-normalised return (optimal_transport.py:143-145)
+    auto max_iter_return = [&](double eps) -> int {
+        int r = finish_outputs(pb, w, eps, 1.0, plan, ldp, row_sums, f_out, g_out, s);
+        if (r) return r;
+        long long nst = 0;
+        WOTK_HIP(hipMemcpyAsync(&nst, w.counter, sizeof(long long), hipMemcpyDeviceToHost, s));
+        WOTK_HIP(hipStreamSynchronize(s));
+        if (stats) {
+            *stats = wotk_stats{};
+            stats->iterations = it;
+            stats->stabilisations = nst;
+            stats->gap_checks = checks;
+            stats->gap = gap;
+            stats->epsilon_final = eps;
+            stats->max_iter_reached = 1;
+        }
+        return WOTK_OK;
+    };
     for (int e = 0; e <= n_scal; ++e) {
         gap = INFINITY;
         // absorb: with a, b folded into the potentials, u = f and v = g
         copy2_kernel<<<64, 256, 0, s>>>(w.u, w.f, pb.n_i, w.v, w.g, pb.n_j);
         WOTK_LAUNCH_CHECK();
         eps_i = eps_i / factor;
-        const double inv_eps = 1.0 / eps_i;
+        inv_eps = 1.0 / eps_i;
         if (e == n_scal) {
             rc = ktilde_sum(pb, w, eps_i, sum_kt, s);
             if (rc) return rc;
         }
-        const double thr = (e == n_scal) ? tolerance : 1e-6;
-        while (gap > thr) {
-            const int64_t nb = (e == n_scal) ? batch_size : 5;
-            for (int64_t t = 0; t < nb; ++t) {
-                ++it;
-                if (e < n_scal && t == nb - 1) {
-                    copy2_kernel<<<64, 256, 0, s>>>(w.fp, w.f, pb.n_i, w.up, w.u, pb.n_i);
-                    copy2_kernel<<<64, 256, 0, s>>>(w.gp, w.g, pb.n_j, w.vp, w.v, pb.n_j);
-                    WOTK_LAUNCH_CHECK();
-                }
-                rc = iterate(pb, w, lambda1, lambda2, eps_i, s);
-                if (rc) return rc;
-                stabilise_kernel<<<1, 1024, 0, s>>>(w.f, w.u, pb.n_i, w.g, w.v, pb.n_j, inv_eps, tau,
-                                                    w.counter);
-                WOTK_LAUNCH_CHECK();
-                if (it >= max_iter) {
-                    // early, un-normalised return (optimal_transport.py:143-145)
-                    rc = finish_outputs(pb, w, eps_i, 1.0, plan, ldp, row_sums, f_out, g_out, s);
-                    if (rc) return rc;
-                    long long nst = 0;
-                    WOTK_HIP(hipMemcpyAsync(&nst, w.counter, sizeof(long long), hipMemcpyDeviceToHost, s));
-                    WOTK_HIP(hipStreamSynchronize(s));
-                    if (stats) {
-                        *stats = wotk_stats{};
-                        stats->iterations = it;
-                        stats->stabilisations = nst;
-                        stats->gap_checks = checks;
-                        stats->gap = gap;
-                        stats->epsilon_final = eps_i;
-                        stats->max_iter_reached = 1;
+        if (e < n_scal) {
+            // dual-variable evolution checks every 5 iterations, threshold 1e-6
+            while (gap > 1e-6) {
+                for (int64_t t = 0; t < 5; ++t) {
+                    ++it;
+                    if (t == 4) {
+                        copy2_kernel<<<64, 256, 0, s>>>(w.fp, w.f, pb.n_i, w.up, w.u, pb.n_i);
+                        copy2_kernel<<<64, 256, 0, s>>>(w.gp, w.g, pb.n_j, w.vp, w.v, pb.n_j);
+                        WOTK_LAUNCH_CHECK();
                     }
-                    return WOTK_OK;
+                    rc = one_iteration(eps_i);
+                    if (rc) return rc;
+                    if (it >= max_iter) return max_iter_return(eps_i);
                 }
-            }
-            ++checks;
-            if (e == n_scal) {
-                rc = gap_pass(pb, w, lambda1, lambda2, eps_i, sum_kt, s);
-                if (rc) return rc;
-                WOTK_HIP(hipMemcpyAsync(host_out, w.gapout, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
-                WOTK_HIP(hipStreamSynchronize(s));
-                gap = host_out[2];
-            } else {
+                ++checks;
                 evolution_kernel<<<1, 1024, 0, s>>>(w.f, w.fp, w.up, w.u, pb.n_i, w.g, w.gp, w.vp, w.v,
                                                     pb.n_j, inv_eps, w.gapout);
                 WOTK_LAUNCH_CHECK();
@@ -767,6 +816,56 @@ extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const flo
                 WOTK_HIP(hipStreamSynchronize(s));
                 gap = host_out[0];
             }
+            continue;
+        }
+        // Last scaling: a primal/dual gap pass after every batch of batch_size
+        // iterations.  The pass (FP64 exp per entry, VALU-bound) runs on a side
+        // stream over a snapshot of the potentials while the next batch (HBM-bound
+        // sweeps) runs ahead on the caller's stream; when the gap has converged,
+        // the batch that ran ahead is rolled back from the snapshot, so the result
+        // is that of the sequential order.  No batch runs ahead when it would
+        // reach max_iter (its early return needs the state at max_iter).
+        SideStream side;
+        WOTK_HIP(hipStreamCreateWithFlags(&side.s, hipStreamNonBlocking));
+        WOTK_HIP(hipEventCreateWithFlags(&side.snap, hipEventDisableTiming));
+        const int snap_blocks = (int)std::min<int64_t>(256, (std::max(pb.n_i, pb.n_j) + 255) / 256);
+        int64_t ahead = 0;  // iterations of the batch launched ahead of the last gap check
+        while (gap > tolerance) {
+            const int64_t nb = batch_size;
+            if (ahead) {
+                it += ahead;  // the batch that ran ahead is this batch
+                ahead = 0;
+            } else {
+                for (int64_t t = 0; t < nb; ++t) {
+                    ++it;
+                    rc = one_iteration(eps_i);
+                    if (rc) return rc;
+                    if (it >= max_iter) return max_iter_return(eps_i);
+                }
+            }
+            ++checks;
+            snapshot_kernel<<<snap_blocks, 256, 0, s>>>(w.f, w.g, w.u, w.v, w.fs, w.gs, w.us, w.vs, pb.n_i,
+                                                        pb.n_j, w.counter, 0);
+            WOTK_LAUNCH_CHECK();
+            WOTK_HIP(hipEventRecord(side.snap, s));
+            WOTK_HIP(hipStreamWaitEvent(side.s, side.snap, 0));
+            rc = gap_pass(pb, w, w.fs, w.gs, lambda1, lambda2, eps_i, sum_kt, side.s);
+            if (rc) return rc;
+            if (it + nb < max_iter) {
+                for (int64_t t = 0; t < nb; ++t) {
+                    rc = one_iteration(eps_i);
+                    if (rc) return rc;
+                }
+                ahead = nb;
+            }
+            WOTK_HIP(hipMemcpyAsync(host_out, w.gapout, 3 * sizeof(double), hipMemcpyDeviceToHost, side.s));
+            WOTK_HIP(hipStreamSynchronize(side.s));
+            gap = host_out[2];
+        }
+        if (ahead) {
+            snapshot_kernel<<<snap_blocks, 256, 0, s>>>(w.f, w.g, w.u, w.v, w.fs, w.gs, w.us, w.vs, pb.n_i,
+                                                        pb.n_j, w.counter, 1);
+            WOTK_LAUNCH_CHECK();
         }
     }
     long long nst = 0;
@@ -816,7 +915,7 @@ extern "C" int wotk_primal_dual(const float *C, int64_t ldc, const float *CT, in
     double *sum_kt = w.misc + 4;
     rc = ktilde_sum(pb, w, epsilon, sum_kt, s);
     if (rc) return rc;
-    rc = gap_pass(pb, w, lambda1, lambda2, epsilon, sum_kt, s);
+    rc = gap_pass(pb, w, w.f, w.g, lambda1, lambda2, epsilon, sum_kt, s);
     if (rc) return rc;
     double host_out[3];
     WOTK_HIP(hipMemcpyAsync(host_out, w.gapout, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
