@@ -120,7 +120,11 @@ int wotk_transport_stablev2(const float *C, int64_t ldc, const float *CT, int64_
                             double *row_sums, double *f_out, double *g_out, void *workspace,
                             size_t workspace_bytes, wotk_stats *stats, void *stream);
 
-/* optimal_transport_duality_gap (wot/ot/optimal_transport.py:67-164). */
+/* optimal_transport_duality_gap (wot/ot/optimal_transport.py:67-164).  The
+ * last epsilon scaling's gap passes run on a second, non-blocking stream that
+ * the call creates and destroys (ordered after the caller's stream by an
+ * event); the call returns with both drained.  Results and counts are those
+ * of the sequential loop. */
 int wotk_transport_duality_gap(const float *C, int64_t ldc, const float *CT, int64_t ldct,
                                int64_t I, int64_t J, const double *G, double lambda1,
                                double lambda2, double epsilon, int64_t batch_size,
