@@ -896,7 +896,8 @@ def bench_validation(args):
     populations (I, R, Rg) and five exact EMDs (I, R, Rg, F, L).  value =
     summaries per second.  The EMDs (network simplex) run on the host; the
     device does PCA, maps, sampling and costs.  The roofline covers the
-    sampling-weight kernels (T read, W written)."""
+    sampling-weight kernels (T read, W written).  The EMDs run on a pool of
+    host threads; the same summary with every EMD in line is timed beside."""
     import torch
     import pandas as pd
     import wot_amd.ot as ot
@@ -934,6 +935,12 @@ def bench_validation(args):
         elapsed = time.perf_counter() - t0
     finally:
         emd_mod.emd2 = real_emd
+    # the same summary with every EMD in line (one host thread), for comparison
+    np.random.seed(1)
+    t0 = time.perf_counter()
+    validation.compute_validation_summary(model, interp_size=c['interp_size'], emd_workers=1)
+    torch.cuda.synchronize()
+    in_line = time.perf_counter() - t0
     # sampling-weight kernels on one 3000 x 3000 map, hipEvents on the current stream
     tmap = model.compute_transport_map(0.0, 2.0, covariate=(1, 1)).X
     Td = torch.from_numpy(tmap).cuda()
@@ -957,7 +964,8 @@ def bench_validation(args):
            'config': {'workload': 'compute_validation_summary, 3 x %d cells x %d genes, local_pca=%d, interp_size=%d'
                                   % (c['cells'], c['genes'], c['local_pca'], c['interp_size']),
                       'rows': list(df['name'])},
-           'emd_host_s': t_emd[0], 'device_and_rest_s': elapsed - t_emd[0],
+           'emd_workers': validation._host_threads(),
+           'emd_host_s_summed_over_threads': t_emd[0], 'summary_s_emds_in_line': in_line,
            'roofline': {'bound': 'hbm', 'kernel': 'colsum_partial_kernel + colsum_final_kernel + weights_kernel',
                         'avg_launch_us': w_us,
                         'achieved': alg / (w_us * 1e-6) / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',

@@ -116,3 +116,16 @@ def test_summary_covariates_growth_and_full(val):
     df2 = val.compute_validation_summary(model, interp_size=40, compute_full_distances=True)
     det = df['name'].isin(['P', 'F', 'L', 'A']).values
     np.testing.assert_allclose(df['distance'].values[det], df2['distance'].values[det], rtol=1e-12)
+
+
+def test_summary_emds_on_host_threads_match_in_line(val):
+    """The EMDs run on a pool of host threads; with the same seed the
+    summary equals the one computed with every EMD in line, row for row."""
+    model, X, obs = _model(covariates=2, growth=True, local_pca=0)
+    np.random.seed(7)
+    a = val.compute_validation_summary(model, interp_size=60, compute_full_distances=True, emd_workers=1)
+    np.random.seed(7)
+    b = val.compute_validation_summary(model, interp_size=60, compute_full_distances=True, emd_workers=4)
+    assert list(a['name']) == list(b['name'])
+    assert b['distance'].dtype == np.float64
+    np.testing.assert_array_equal(a['distance'].values, b['distance'].values)

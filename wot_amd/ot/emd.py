@@ -40,9 +40,10 @@ def _dense(x):
     return x.toarray() if scipy.sparse.issparse(x) else np.asarray(x)
 
 
-def earth_mover_distance(cloud1, cloud2, eigenvals=None, weights1=None, weights2=None):
-    """sqrt(EMD) between two point clouds under the squared-Euclidean cost
-    (util.py:202-237); ``eigenvals`` scales both clouds first."""
+def emd_inputs(cloud1, cloud2, eigenvals=None, weights1=None, weights2=None):
+    """(p, q, M) of earth_mover_distance: the clouds' weights (uniform unless
+    given) and their squared-Euclidean cost, built on the device, as host
+    arrays (util.py:202-237)."""
     c1, c2 = _dense(cloud1), _dense(cloud2)
     if eigenvals is not None:
         c1 = c1.dot(eigenvals)
@@ -50,4 +51,11 @@ def earth_mover_distance(cloud1, cloud2, eigenvals=None, weights1=None, weights2
     p = np.ones(len(c1)) / len(c1) if weights1 is None else np.asarray(weights1, np.float64) / np.sum(weights1)
     q = np.ones(len(c2)) / len(c2) if weights2 is None else np.asarray(weights2, np.float64) / np.sum(weights2)
     M = _dev.sqeuclid_f64(c1, c2).cpu().numpy()
+    return p, q, M
+
+
+def earth_mover_distance(cloud1, cloud2, eigenvals=None, weights1=None, weights2=None):
+    """sqrt(EMD) between two point clouds under the squared-Euclidean cost
+    (util.py:202-237); ``eigenvals`` scales both clouds first."""
+    p, q, M = emd_inputs(cloud1, cloud2, eigenvals, weights1, weights2)
     return np.sqrt(emd2(p, q, M, numItermax=int(1e7)))

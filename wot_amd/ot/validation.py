@@ -8,9 +8,15 @@ weights, inverse-CDF sampling, gather + lerp).  The uniforms are drawn from
 numpy's global generator exactly as the reference's ``np.random.choice``
 draws them: ``random_sample(size)`` with weights, ``randint(0, I*J, size)``
 without.  A seeded run therefore picks the same cell pairs.  EMDs use
-``wot_amd.ot.emd`` (device cost, exact network simplex).
+``wot_amd.ot.emd`` (device cost, exact network simplex); the summary's EMDs
+are independent, so their network simplex runs on a pool of host threads
+(the library call releases the GIL) while the main thread goes on with the
+device work and the random draws in the reference's order.
 """
+import collections
 import itertools
+import os
+from concurrent.futures import Future, ThreadPoolExecutor
 
 import numpy as np
 import pandas as pd
@@ -18,7 +24,7 @@ import scipy.sparse
 
 from .. import _anndata, _native
 from . import device as _dev
-from .emd import earth_mover_distance
+from . import emd as _emd
 from .pca import compute_pca
 
 
@@ -107,6 +113,40 @@ def interpolate_randomly_with_growth(p0, p1, t, size, g):
     return _sample_rows(p0, p1, choices, t)
 
 
+def _host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+class _EmdRunner:
+    """Exact EMDs of one summary: costs built on the device in call order
+    (calling thread), network simplex on ``workers`` host threads; at most
+    2 x workers cost matrices are held at once.  ``submit`` returns the
+    distance (one worker) or a Future of it."""
+
+    def __init__(self, workers):
+        self.pool = ThreadPoolExecutor(workers) if workers > 1 else None
+        self.limit = 2 * workers
+        self.pending = collections.deque()
+
+    def submit(self, cloud1, cloud2, eigenvals):
+        p, q, M = _emd.emd_inputs(cloud1, cloud2, eigenvals)
+        if self.pool is None:
+            return float(np.sqrt(_emd.emd2(p, q, M, numItermax=int(1e7))))
+        fut = self.pool.submit(lambda: float(np.sqrt(_emd.emd2(p, q, M, numItermax=int(1e7)))))
+        self.pending.append(fut)
+        while len(self.pending) > self.limit:
+            self.pending.popleft().result()
+        return fut
+
+    def close(self):
+        if self.pool is not None:
+            self.pool.shutdown(wait=True)
+
+
 def split_anndata(ds, field):
     """{value: rows of ds with obs[field] == value} in order of appearance (wot.split_anndata)."""
     out = {}
@@ -115,11 +155,24 @@ def split_anndata(ds, field):
     return out
 
 
-def compute_validation_summary(ot_model, day_triplets=None, interp_size=10000, compute_full_distances=False):
+def compute_validation_summary(ot_model, day_triplets=None, interp_size=10000, compute_full_distances=False,
+                               emd_workers=None):
     """Validation summary (optimal_transport_validation.py:15-196): one row
     per distance (interval_start, interval_mid, interval_end, t0, t1, cv0,
     cv1, name in P / I / F / L / R / Rg (/ I1 / I2 / A when full), distance,
-    full)."""
+    full).  ``emd_workers``: host threads for the EMDs (default: the cores
+    this process may use, at most 16; 1 runs them in line)."""
+    runner = _EmdRunner(_host_threads() if emd_workers is None else max(1, int(emd_workers)))
+    try:
+        df = _summary(ot_model, day_triplets, interp_size, compute_full_distances, runner)
+    finally:
+        runner.close()
+    if 'distance' in df:
+        df['distance'] = [d.result() if isinstance(d, Future) else d for d in df['distance']]
+    return df
+
+
+def _summary(ot_model, day_triplets, interp_size, compute_full_distances, runner):
     if ot_model.covariate_field not in ot_model.matrix.obs:
         ot_model.matrix.obs['covariate'] = 1
     if day_triplets is None:
@@ -167,7 +220,7 @@ def compute_validation_summary(ot_model, day_triplets=None, interp_size=10000, c
             tmap_full = ot_model.compute_transport_map(t0, t1)
 
             def full(pop, t, name, pop2=p05_ds.X):
-                record(earth_mover_distance(pop, pop2, eigenvals), t, name, '', '', True)
+                record(runner.submit(pop, pop2, eigenvals), t, name, '', '', True)
 
             if growth_field in obs_all:
                 g = p0_ds.obs[growth_field].values ** interp_frac
@@ -193,7 +246,7 @@ def compute_validation_summary(ot_model, day_triplets=None, interp_size=10000, c
             seen_first, seen_last = set(), set()
 
             def to_p05(pop, t, name, cv):
-                record(earth_mover_distance(pop, p05_x, eigenvals), t, name, cv, cv05, False)
+                record(runner.submit(pop, p05_x, eigenvals), t, name, cv, cv05, False)
 
             for cv05_2 in p05.keys():
                 if cv05_2 != cv05:
