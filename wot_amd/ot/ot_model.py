@@ -129,7 +129,10 @@ class OTModel:
         """Computes and saves every consecutive (or configured) day pair
         (ot_model.py:124-201).  Existing outputs are skipped when
         ``overwrite`` is False; learned growth goes to ``<prefix>_g.txt``
-        when ``growth_iters > 1``."""
+        when ``growth_iters > 1``.  Each map is written by one background
+        thread while the next pair is solved (at most one write in flight, so
+        one extra host copy of a map); write errors surface here."""
+        from concurrent.futures import ThreadPoolExecutor
         from ..io import check_file_extension, write_dataset
         tmap_dir, tmap_prefix = os.path.split(tmap_out) if tmap_out is not None else (None, None)
         tmap_prefix = tmap_prefix or "tmaps"
@@ -149,25 +152,40 @@ class OTModel:
             cost_matrices = [None for _ in day_pairs]
         full_learned_growth_df = None
         save_learned_growth = self.ot_config.get('growth_iters', 1) > 1
-        for day_pair, cost_matrix in zip(day_pairs, cost_matrices):
-            path = tmap_prefix
-            if not with_covariates:
-                path += "_{}_{}".format(*day_pair)
-            else:
-                path += "_{}_{}_cv{}_cv{}".format(*day_pair)
-            output_file = check_file_extension(os.path.join(tmap_dir, path), output_file_format)
-            if os.path.exists(output_file) and not overwrite:
-                logger.info('Found existing tmap at ' + output_file + '. ')
-                continue
-            tmap = self.compute_transport_map(*day_pair, cost_matrix=cost_matrix)
-            write_dataset(tmap, output_file, output_format=output_file_format)
-            if save_learned_growth:
-                learned_growth_df = tmap.obs
-                full_learned_growth_df = learned_growth_df if full_learned_growth_df is None else pd.concat(
-                    (full_learned_growth_df, learned_growth_df), copy=False)
+        with ThreadPoolExecutor(1) as writer:
+            pending = None
+            for day_pair, cost_matrix in zip(day_pairs, cost_matrices):
+                tmap, output_file = self._next_map(day_pair, cost_matrix, with_covariates, tmap_dir, tmap_prefix,
+                                                   output_file_format, overwrite, check_file_extension)
+                if tmap is None:
+                    continue
+                if pending is not None:
+                    pending.result()
+                pending = writer.submit(write_dataset, tmap, output_file, output_format=output_file_format)
+                if save_learned_growth:
+                    learned_growth_df = tmap.obs
+                    full_learned_growth_df = learned_growth_df if full_learned_growth_df is None else pd.concat(
+                        (full_learned_growth_df, learned_growth_df), copy=False)
+            if pending is not None:
+                pending.result()
         if full_learned_growth_df is not None:
             full_learned_growth_df.to_csv(os.path.join(tmap_dir, tmap_prefix + '_g.txt'), sep='\t',
                                           index_label='id')
+
+    def _next_map(self, day_pair, cost_matrix, with_covariates, tmap_dir, tmap_prefix, output_file_format,
+                  overwrite, check_file_extension):
+        """(map, output path) of one pair of compute_all_transport_maps, or
+        (None, path) when the file exists and overwrite is off."""
+        path = tmap_prefix
+        if not with_covariates:
+            path += "_{}_{}".format(*day_pair)
+        else:
+            path += "_{}_{}_cv{}_cv{}".format(*day_pair)
+        output_file = check_file_extension(os.path.join(tmap_dir, path), output_file_format)
+        if os.path.exists(output_file) and not overwrite:
+            logger.info('Found existing tmap at ' + output_file + '. ')
+            return None, output_file
+        return self.compute_transport_map(*day_pair, cost_matrix=cost_matrix), output_file
 
     def compute_transport_map(self, t0, t1, covariate=None, cost_matrix=None):
         """Transport map from t0 to t1 (ot_model.py:203-240)."""
