@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
             const double uc = (h_cur - n_cur) * fa.k;
 #pragma unroll
             for (int r = 0; r < 16; ++r)
-                s[r] += __builtin_amdgcn_exp2f((float)fma(k2, (double)acc[r] + (double)acc2[r], uc + wf[r]));
+                s[r] += __builtin_amdgcn_exp2f((float)fma(k2, (double)(acc[r] + acc2[r]), uc + wf[r]));
         }
         if (t + 1 < ntiles) commit(buf ^ 1);  // the other buffer was released by the previous barrier
         h_cur = h_nxt;
