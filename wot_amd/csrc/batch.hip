@@ -619,7 +619,7 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 
 #pragma unroll
                         for (int l = 0; l < NL; ++l)
                             if (l != L0) {
-                                s[r][g * NL + l] += fmaf(tq[0], bf[i].x, fmaf(tq[1], bf[i].y, fmaf(tq[2], bf[i].z, tq[3] * bf[i].w)));
+                                s[r][g * NL + l] = fmaf(tq[3], bf[i].w, fmaf(tq[2], bf[i].z, fmaf(tq[1], bf[i].y, fmaf(tq[0], bf[i].x, s[r][g * NL + l]))));
                                 ++i;
                             }
                     }
