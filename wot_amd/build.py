@@ -21,6 +21,13 @@ DEVICE_FLAGS = [
 ]
 
 
+# per-source device flags: the never-stored sweep keeps its MFMA accumulators
+# in ordinary VGPRs (no AGPR copies before the per-entry exponentials)
+SOURCE_FLAGS = {
+    'fused.hip': ['-mllvm', '-amdgpu-mfma-vgpr-form'],
+}
+
+
 def sources():
     return sorted(glob.glob(os.path.join(CSRC, '*.hip')) + glob.glob(os.path.join(CSRC, '*.cpp')))
 
@@ -32,7 +39,7 @@ def build(verbose=False, jobs=8):
     for src in sources():
         obj = os.path.join(LIBDIR, os.path.basename(src) + '.o')
         lang = ['-x', 'hip'] if src.endswith('.hip') else []
-        cmd = [HIPCC] + DEVICE_FLAGS + lang + ['-c', src, '-o', obj]
+        cmd = [HIPCC] + DEVICE_FLAGS + SOURCE_FLAGS.get(os.path.basename(src), []) + lang + ['-c', src, '-o', obj]
         if src.endswith('.cpp'):
             cmd = [HIPCC, '-O3', '-std=c++17', '-fPIC', '-c', src, '-o', obj]
         objs.append(obj)
