@@ -23,6 +23,8 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kFThreads = 256;
+constexpr int kFusedMaxSplit = 8;        // column splits per sweep, at most
+constexpr int kFusedTargetBlocks = 768;  // 3 waves per SIMD x 1024 SIMDs / 4 waves per block
 
 // ---------------------------------------------------------------------------
 // preparation
@@ -194,7 +196,26 @@ struct FusedArgs {
     int *bad_list;               // rows whose fixed-shift sum under/overflowed
     int *n_bad;
     double *row_sums;            // row-sum mode: write sum_j R_ij / n_cols, leave pot unchanged
+    double *part;                // column split (gridDim.y > 1): per-split row sums, split-major
 };
+
+// Row epilogue of a sweep from the row's fixed-shift sum S: new potential
+// (or the plan's row sum in row-sum mode); rows whose sum under/overflowed
+// are queued for the repair kernel (a NaN sum leaves the row as it is).
+__device__ inline void fused_row_epilogue(const FusedArgs &fa, int row, double S) {
+    const double f_old = fa.pot[row];
+    if (!(S > 0.0) || isinf(S)) {
+        if (!isnan(S)) fa.bad_list[atomicAdd(fa.n_bad, 1)] = row;
+        return;
+    }
+    if (fa.row_sums) {  // the shift is the final potential: S = sum_j R_ij
+        fa.row_sums[row] = S / fa.n_cols;
+        return;
+    }
+    const double lse = log(S) - f_old / fa.eps;  // ln sum_j exp((h_j - C_ij)/eps)
+    const double lw = fa.logw ? fa.logw[row] : *fa.logw_dev;
+    fa.pot[row] = fa.alpha * fa.eps * (lw + fa.log_n_other - lse);
+}
 
 template <int TERMS, int NKS>
 __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
@@ -259,20 +280,24 @@ __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
         }
     };
 
-    const int ntiles = (fa.n_cols + 31) / 32;
+    // column split: block (x, y) takes the column tiles [t_begin, t_end) of its rows
+    const int ntiles_all = (fa.n_cols + 31) / 32;
+    const int per_split = (ntiles_all + (int)gridDim.y - 1) / (int)gridDim.y;
+    const int t_begin = min(ntiles_all, (int)blockIdx.y * per_split);
+    const int t_end = min(ntiles_all, t_begin + per_split);
     auto col_const = [&](int t, double &hv, double &nv) {  // this lane's column of tile t
         const int col = min(t * 32 + lr, fa.n_cols - 1);
         hv = fa.h[col];
         nv = fa.nb[col];
     };
     double h_cur, n_cur, h_nxt = 0.0, n_nxt = 0.0;
-    col_const(0, h_cur, n_cur);
-    fetch(0);
+    col_const(t_begin, h_cur, n_cur);
+    fetch(t_begin * 32);
     commit(0);
     __syncthreads();
-    for (int t = 0; t < ntiles; ++t) {
-        const int buf = t & 1;
-        if (t + 1 < ntiles) {
+    for (int t = t_begin; t < t_end; ++t) {
+        const int buf = (t - t_begin) & 1;
+        if (t + 1 < t_end) {
             fetch((t + 1) * 32);
             col_const(t + 1, h_nxt, n_nxt);
         }
@@ -306,7 +331,7 @@ __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
             for (int r = 0; r < 16; ++r)
                 s[r] += __builtin_amdgcn_exp2f((float)fma(k2, (double)(acc[r] + acc2[r]), uc + wf[r]));
         }
-        if (t + 1 < ntiles) commit(buf ^ 1);  // the other buffer was released by the previous barrier
+        if (t + 1 < t_end) commit(buf ^ 1);  // the other buffer was released by the previous barrier
         h_cur = h_nxt;
         n_cur = n_nxt;
         __syncthreads();
@@ -325,21 +350,20 @@ __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
         for (int r = 0; r < 16; ++r) {
             const int row = r0 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
             if (row >= fa.n_rows) continue;
-            const double S = sd[r];
-            const double f_old = fa.pot[row];
-            if (!(S > 0.0) || isinf(S)) {
-                if (!isnan(S)) fa.bad_list[atomicAdd(fa.n_bad, 1)] = row;
-                continue;
-            }
-            if (fa.row_sums) {  // the shift is the final potential: S = sum_j R_ij
-                fa.row_sums[row] = S / fa.n_cols;
-                continue;
-            }
-            const double lse = log(S) - f_old / fa.eps;  // ln sum_j exp((h_j - C_ij)/eps)
-            const double lw = fa.logw ? fa.logw[row] : *fa.logw_dev;
-            fa.pot[row] = fa.alpha * fa.eps * (lw + fa.log_n_other - lse);
+            if (gridDim.y > 1) fa.part[(int64_t)blockIdx.y * fa.n_rows + row] = sd[r];
+            else fused_row_epilogue(fa, row, sd[r]);
         }
     }
+}
+
+// column-split sweeps: the splits' row sums added in a fixed order, then the
+// row epilogue
+__global__ __launch_bounds__(256) void fused_finalize_kernel(FusedArgs fa, int n_split) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= fa.n_rows) return;
+    double S = 0.0;
+    for (int q = 0; q < n_split; ++q) S += fa.part[(int64_t)q * fa.n_rows + row];
+    fused_row_epilogue(fa, row, S);
 }
 
 // repair: rows whose fixed-shift sum under/overflowed, from FP64 direct
@@ -428,6 +452,7 @@ struct FzWork {
     double *mean, *xs, *ys, *nx, *ny, *f, *g, *logp, *misc;
     __bf16 *xp[3], *yp[3];
     int *bad, *nbad;
+    double *part;
     FzSelect *sel;
     unsigned long long *hist;
 };
@@ -449,6 +474,7 @@ static size_t fz_carve(Arena &ar, int64_t I, int64_t J, int64_t d, FzWork *w) {
     }
     w->bad = ar.take<int>(std::max(I, J));
     w->nbad = ar.take<int>(4);
+    w->part = ar.take<double>((size_t)kFusedMaxSplit * std::max(I, J));
     w->sel = ar.take<FzSelect>(1);
     w->hist = ar.take<unsigned long long>(2048);
     return ar.off;
@@ -457,9 +483,18 @@ static size_t fz_carve(Arena &ar, int64_t I, int64_t J, int64_t d, FzWork *w) {
 static const int kFzShift[6] = {53, 42, 31, 20, 10, 0};
 static const int kFzWidth[6] = {11, 11, 11, 11, 10, 10};
 
+// column splits per sweep: enough blocks for ~3 waves per SIMD (the kernel
+// holds 166 VGPRs), at least 8 column tiles per split
+static int fused_splits(int n_rows, int n_cols) {
+    const int blocks = (n_rows + 127) / 128;
+    const int ntiles = (n_cols + 31) / 32;
+    const int want = (kFusedTargetBlocks + blocks - 1) / blocks;
+    return std::max(1, std::min(std::min(want, kFusedMaxSplit), ntiles / 8));
+}
+
 template <int TERMS>
-static void launch_fused(const FusedArgs &fa, hipStream_t s) {
-    const unsigned grid = (unsigned)((fa.n_rows + 127) / 128);
+static void launch_fused(const FusedArgs &fa, int n_split, hipStream_t s) {
+    const dim3 grid((unsigned)((fa.n_rows + 127) / 128), (unsigned)n_split);
     switch (fa.kp / 16) {
         case 1: fused_lse_kernel<TERMS, 1><<<grid, kFThreads, 0, s>>>(fa); break;
         case 2: fused_lse_kernel<TERMS, 2><<<grid, kFThreads, 0, s>>>(fa); break;
@@ -475,7 +510,9 @@ static void launch_fused(const FusedArgs &fa, hipStream_t s) {
 template <int TERMS>
 static int fused_sweep(const FusedArgs &fa, const double *xr, const double *xc, int d, hipStream_t s) {
     fused_clear_kernel<<<1, 1, 0, s>>>(fa.n_bad);
-    launch_fused<TERMS>(fa, s);
+    const int n_split = fused_splits(fa.n_rows, fa.n_cols);
+    launch_fused<TERMS>(fa, n_split, s);
+    if (n_split > 1) fused_finalize_kernel<<<(fa.n_rows + 255) / 256, 256, 0, s>>>(fa, n_split);
     fused_repair_kernel<<<64, 256, 0, s>>>(fa.bad_list, fa.n_bad, xr, xc, fa.n_cols, d, fa.h, fa.pot, fa.logw,
                                            fa.logw_dev, fa.log_n_other, fa.alpha, fa.eps, fa.k, fa.row_sums);
     WOTK_LAUNCH_CHECK();
@@ -550,9 +587,9 @@ extern "C" int wotk_transport_stablev2_coords(const double *X, const double *Y, 
         eps = it < scaling_iter ? reg(it / inner_iter_max) : reg(scaling_iter / inner_iter_max);
         const double k = kLog2E / eps;
         FusedArgs fr{w.xp[0], w.xp[1], w.xp[2], w.yp[0], w.yp[1], w.yp[2], w.nx, w.ny, (int)I, (int)J, kp,
-                     w.g, w.f, w.logp, nullptr, lnJ, lambda1 / (lambda1 + eps), eps, k, w.bad, w.nbad, nullptr};
+                     w.g, w.f, w.logp, nullptr, lnJ, lambda1 / (lambda1 + eps), eps, k, w.bad, w.nbad, nullptr, w.part};
         FusedArgs fc{w.yp[0], w.yp[1], w.yp[2], w.xp[0], w.xp[1], w.xp[2], w.ny, w.nx, (int)J, (int)I, kp,
-                     w.f, w.g, nullptr, w.misc, lnI, lambda2 / (lambda2 + eps), eps, k, w.bad, w.nbad, nullptr};
+                     w.f, w.g, nullptr, w.misc, lnI, lambda2 / (lambda2 + eps), eps, k, w.bad, w.nbad, nullptr, w.part};
         int rc = terms == 3 ? fused_sweep<3>(fr, w.xs, w.ys, (int)d, s) : fused_sweep<6>(fr, w.xs, w.ys, (int)d, s);
         if (rc) return rc;
         rc = terms == 3 ? fused_sweep<3>(fc, w.ys, w.xs, (int)d, s) : fused_sweep<6>(fc, w.ys, w.xs, (int)d, s);
@@ -563,7 +600,7 @@ extern "C" int wotk_transport_stablev2_coords(const double *X, const double *Y, 
         // the returned plan is built with the eps after the last iteration
         const double e_end = reg(scaling_iter / inner_iter_max);
         FusedArgs fr{w.xp[0], w.xp[1], w.xp[2], w.yp[0], w.yp[1], w.yp[2], w.nx, w.ny, (int)I, (int)J, kp,
-                     w.g, w.f, w.logp, nullptr, lnJ, 0.0, e_end, kLog2E / e_end, w.bad, w.nbad, row_sums};
+                     w.g, w.f, w.logp, nullptr, lnJ, 0.0, e_end, kLog2E / e_end, w.bad, w.nbad, row_sums, w.part};
         int rc = terms == 3 ? fused_sweep<3>(fr, w.xs, w.ys, (int)d, s) : fused_sweep<6>(fr, w.xs, w.ys, (int)d, s);
         if (rc) return rc;
     }
