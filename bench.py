@@ -1015,7 +1015,7 @@ def main():
                     help='timecourse / large: override scaling_iter (extra_iter scaled alike) for quick probes')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py measures one GPU: this build has no multi-GPU path (got --gpus %d)' % args.gpus,
+        print('bench.py measures one GPU: libwotk has no multi-GPU path (got --gpus %d)' % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     if args.config == 'gap':
