@@ -216,7 +216,11 @@ def bench_timecourse(args):
         batch.solve_time_course(coords[:3], growth[:3], scaling_iter=2, extra_iter=0, **GRID)
     torch.cuda.synchronize()
     tiles = None
-    if not args.no_tune:
+    if args.tile:
+        from wot_amd import tune
+        tune.set_tile(tune.BATCH_SHARED, -1, _parse_tile(args.tile))
+        tiles = {'kernel': 'batch_shared', 'tile': dict(zip(('rows', 'threads'), _parse_tile(args.tile)))}
+    elif not args.no_tune:
         from wot_amd import tune
         tc = batch.prepare_time_course(coords, growth)
         tiles = tune.autotune_time_course(tc, GRID['epsilons'], GRID['lambda1s'], lambda2=sched['lambda2'],
@@ -254,6 +258,8 @@ def bench_timecourse(args):
                      'achieved': alg_bytes / avg_sweep / 1e9, 'peak': HBM_PEAK_GBS,
                      'frac': alg_bytes / avg_sweep / 1e9 / HBM_PEAK_GBS, 'traffic': None},
     }
+    if args.no_parity:
+        return out
     # parity: a few (pair, combination) maps against the single-pair solver
     # (itself checked against the FP64 oracle) on the same device cost
     worst = 0.0
@@ -272,6 +278,11 @@ def bench_timecourse(args):
     if not args.no_baselines:
         out.update(_timecourse_baselines(torch, coords, growth, pair_ij, n_combo, sched, n_iter, elapsed))
     return out
+
+
+def _parse_tile(text):
+    r, t = (int(v) for v in text.lower().split('x'))
+    return r, t
 
 
 def _tc_kernel_name(stats, tiles):
@@ -1013,6 +1024,10 @@ def main():
                     help='skip the tile autotuner (use persisted or default tiles)')
     ap.add_argument('--iters', type=int, default=None,
                     help='timecourse / large: override scaling_iter (extra_iter scaled alike) for quick probes')
+    ap.add_argument('--tile', default=None,
+                    help='timecourse: force the shared-exponential sweep tile ROWSxTHREADS (no tuning)')
+    ap.add_argument('--no-parity', action='store_true',
+                    help='timecourse: skip the parity and cost-error legs (profiling runs)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: libwotk has no multi-GPU path (got --gpus %d)' % args.gpus,

@@ -65,3 +65,68 @@ def read_dataset(path):
         return _anndata.AnnData(df.values.astype(np.float64), pd.DataFrame(index=df.index.astype(str)),
                                 pd.DataFrame(index=df.columns.astype(str)))
     raise ValueError('Unknown file format for %s (supported: .h5ad, .npz, .txt, .tsv, .csv)' % p)
+
+
+# ---- cell / gene filters (wot/io/io.py:499-518 filter_adata) ---------------
+def _set_file_ext(path):
+    """Extension of a gene-set file, with '.gmt.txt' / '.grp.txt' / '.gmx.txt'
+    read as the inner format (io.py:480-495)."""
+    name = str(path).rsplit('/', 1)[-1].lower()
+    parts = name.split('.')
+    if len(parts) >= 3 and parts[-1] == 'txt' and parts[-2] in ('gmt', 'grp', 'gmx'):
+        return parts[-2]
+    return parts[-1] if len(parts) > 1 else ''
+
+
+def read_set_ids(path):
+    """Every id listed in a set file (the obs index of wot.io.read_sets(path),
+    io.py:183-204): grp / txt (one id per line; lines starting with '#' or '>'
+    skipped), gmt (tab-separated: name, description, members) or gmx (one set
+    per column: name, description, then members).  Ids are kept in first-seen
+    order, de-duplicated case-insensitively as the reference's reader does."""
+    ext = _set_file_ext(path)
+    ids = []
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    if ext in ('txt', 'grp'):
+        ids = [ln.strip() for ln in lines if ln and ln[0] not in '#>' and ln.strip()]
+    elif ext == 'gmt':
+        for ln in lines:
+            fields = ln.rstrip('\n').split('\t')
+            if len(fields) > 2:
+                ids.extend(v.strip() for v in fields[2:] if v.strip())
+    elif ext == 'gmx':
+        rows = [ln.split('\t') for ln in lines]
+        for r in rows[2:]:
+            ids.extend(v.strip() for v in r if v.strip())
+    else:
+        raise ValueError('Unknown file format "{}"'.format(ext))
+    seen = {}
+    for v in ids:
+        seen.setdefault(v.lower(), v)
+    return list(seen.values())
+
+
+def _filter_mask(index, frame, spec):
+    """Boolean keep-mask over ``index`` for one filter spec: an existing file
+    path (its ids), a comma-separated id list, one boolean column name of
+    ``frame``, or (beyond the reference) any non-string iterable of ids."""
+    import os
+    if isinstance(spec, str):
+        if os.path.exists(spec):
+            return np.asarray(index.isin(read_set_ids(spec)))
+        names = spec.split(',')
+        if len(names) == 1 and names[0] in frame:
+            return np.asarray(frame[names[0]] == True)  # noqa: E712 (the column may hold 0/1)
+        return np.asarray(index.isin(names))
+    return np.asarray(index.isin(list(spec)))
+
+
+def filter_adata(adata, obs_filter=None, var_filter=None):
+    """Keep the cells (``obs_filter``) and genes (``var_filter``) a filter
+    selects (io.py:499-518)."""
+    if obs_filter is not None:
+        adata = adata[_filter_mask(adata.obs.index, adata.obs, obs_filter)].copy()
+    if var_filter is not None:
+        adata = adata[:, _filter_mask(adata.var.index, adata.var, var_filter)].copy()
+    return adata

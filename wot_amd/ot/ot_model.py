@@ -14,6 +14,7 @@ import numpy as np
 import pandas as pd
 
 from .. import _anndata
+from ..io import filter_adata
 from . import device as _dev
 from . import optimal_transport as _ot
 from .initializer import parse_configuration, parse_parameter_file
@@ -54,12 +55,7 @@ class OTModel:
         ncounts = kwargs.pop('ncounts', None)
         ncells = kwargs.pop('ncells', None)
         self.cost_mode = kwargs.pop('cost_mode', 'f64')
-        if cell_filter is not None:
-            keep = self.matrix.obs.index.isin(list(cell_filter))
-            self.matrix = self.matrix[keep, :]
-        if gene_filter is not None:
-            keep = self.matrix.var.index.isin(list(gene_filter))
-            self.matrix = self.matrix[:, keep]
+        self.matrix = filter_adata(self.matrix, obs_filter=cell_filter, var_filter=gene_filter)
         if day_filter is not None:
             days = [float(day) for day in day_filter.split(',')] if isinstance(day_filter, str) else day_filter
             self.matrix = self.matrix[self.matrix.obs[self.day_field].isin(days).values].copy()
