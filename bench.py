@@ -2,17 +2,17 @@
 solver on one MI355X, versus the plain-GPU torch FP64 restatement and the
 FP64 CPU oracle.
 
-Workload (one "step" = one transport map, as OTModel.compute_transport_map
-computes it): one day pair of 15,000 x 15,000 cells, 30-d PCA coordinates from
-a seeded 10-component Gaussian mixture (target means shifted by N(0, 0.5)),
-log-normal growth; cost = squared Euclidean / median on the device, cross term
-as a 6-term split-BF16 MFMA GEMM with FP32 accumulation and FP64 norms
-(``--cost-mode``; the 3-term split misses the 1e-5 plan gate, see the
-``cost_split_bf16`` report);
-fixed_iters solver (eps=0.05, lambda1=1, lambda2=50, eps0=1, tau=1e4,
-scaling_iter=3000, inner_iter_max=50, extra_iter=1000) with growth_iters=3,
-i.e. 12,000 Sinkhorn iterations per map; the FP64 plan is materialised in HBM.
-Inputs are resident in HBM before the timed region.
+Default workload (``--config timecourse``): a 38-pair time course (39 days,
+cells per day ~ U[1000, 20000], 30-d PCA coordinates from a seeded 20-component
+Gaussian mixture drifting in time) swept over eps {0.01, 0.05, 0.1} x lambda1
+{0.5, 1, 5}: 342 fixed_iters solves (lambda2=50, eps0=1, scaling_iter=3000,
+inner_iter_max=50, extra_iter=1000) advancing together in one persistent
+launch per half-iteration.  One step = 200 iterations of every solve; 20
+steps = one complete course, costs and medians built on the device in the
+first step and every FP64 map written once in HBM after the last.  Inputs
+(coordinates, growth) are resident in HBM before the timed region.
+``--config pair`` times whole 15,000 x 15,000 maps (growth_iters=3) instead;
+the other configs are the remaining workloads.
 
 Prints ONE JSON line.  Usage:  python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -190,13 +190,55 @@ TC = dict(n_days=39, min_cells=1000, max_cells=20000, dim=30, n_components=20, s
 GRID = dict(epsilons=(0.01, 0.05, 0.1), lambda1s=(0.5, 1.0, 5.0))
 
 
+STEP_ITERS = 200  # iterations of every solve per step: 20 steps = one full 3000 + 1000 schedule
+
+
+def host_info():
+    """Host cores and CPU model (for the CPU baseline)."""
+    model = None
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for line in fh:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:  # pragma: no cover
+        pass
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get('num_threads', 1) for i in threadpool_info()] + [1])
+    except Exception:  # pragma: no cover
+        threads = os.cpu_count()
+    return {'nproc': os.cpu_count(), 'blas_threads': threads, 'cpu_model': model}
+
+
+def valu_rates(torch):
+    """Issue rates of the sweep's instructions on this device (libwotk probe)."""
+    import ctypes
+    from wot_amd import _native
+    from wot_amd.ot import device as dev
+    L = _native.lib()
+    sink = torch.empty(L.wotk_probe_sink_bytes() // 4, dtype=torch.float32, device='cuda')
+    rates = (ctypes.c_double * 7)()
+    _native.check(L.wotk_probe_valu_rates(rates, 7, dev._ptr(sink), sink.numel() * 4, dev._stream(torch)))
+    names = ['exp_f32', 'fma_f64', 'fma_f32', 'cvt_f64_f32', 'rndne_f64_add', 'ldexp_f32', 'split_exp_entry']
+    return {n: float(v) for n, v in zip(names, rates)}
+
+
 def bench_timecourse(args):
     """38 consecutive day pairs (cells per day ~ U[1000, 20000], 30-d, 20
     drifting components) x the grid eps {0.01, 0.05, 0.1} x lambda1 {0.5, 1, 5}
-    = 342 fixed_iters solves in one batched run (hipGraph replay of 50-iteration
-    blocks).  value = solves (transport maps) per second."""
+    = 342 fixed_iters solves advancing together (one persistent launch per
+    half-iteration, hipGraph replay of 50-iteration blocks).
+
+    A step is STEP_ITERS (200) Sinkhorn iterations of all 342 solves, so 20
+    steps are one complete time course: every pair's cost and median built on
+    the device from coordinates already in HBM (in the first step), the full
+    3000 + 1000 schedule, the final potentials, and every FP64 map
+    materialised in HBM once (one at a time into a reused buffer, after the
+    last step).  value = solves per second = 342 x (steps x 200 / 4000) /
+    elapsed, i.e. exactly 342 / elapsed at the default 20 steps."""
     import torch
-    import wot_amd.ot as ot
     from wot_amd.ot import batch
     from wot_amd.synthetic import time_course
     coords, growth, counts = time_course(**TC)
@@ -208,13 +250,19 @@ def bench_timecourse(args):
         sched['extra_iter'] = sched['extra_iter'] * args.iters // sched['scaling_iter']
         sched['scaling_iter'] = args.iters
     n_iter = sched['scaling_iter'] + sched['extra_iter']
+    step_iters = min(STEP_ITERS, n_iter)
+    steps_per_course = -(-n_iter // step_iters)
     pair_ij = [int(counts[t]) * int(counts[t + 1]) for t in range(len(counts) - 1)]
     sum_ij = float(sum(pair_ij))
     n_combo = len(GRID['epsilons']) * len(GRID['lambda1s'])
-    if args.warmup:
-        log('warmup (2 iterations)')
-        batch.solve_time_course(coords[:3], growth[:3], scaling_iter=2, extra_iter=0, **GRID)
-    torch.cuda.synchronize()
+    n_solves = len(pair_ij) * n_combo
+    # inputs resident in HBM before anything is timed
+    dcoords = [torch.from_numpy(np.ascontiguousarray(c)).cuda() for c in coords]
+    dgrowth = [torch.from_numpy(np.ascontiguousarray(g)).cuda() for g in growth]
+    combos, eps_a, l1_a, l2_a = batch.grid_arrays(len(pair_ij), GRID['epsilons'], GRID['lambda1s'],
+                                                  sched['lambda2'])
+    run_kw = dict(scaling_iter=sched['scaling_iter'], epsilon0=sched['epsilon0'], extra_iter=sched['extra_iter'],
+                  inner_iter_max=sched['inner_iter_max'], graph_iters=50, combos=combos)
     tiles = None
     if args.tile:
         from wot_amd import tune
@@ -222,62 +270,139 @@ def bench_timecourse(args):
         tiles = {'kernel': 'batch_shared', 'tile': dict(zip(('rows', 'threads'), _parse_tile(args.tile)))}
     elif not args.no_tune:
         from wot_amd import tune
-        tc = batch.prepare_time_course(coords, growth)
+        tc = batch.prepare_time_course(dcoords, dgrowth)
         tiles = tune.autotune_time_course(tc, GRID['epsilons'], GRID['lambda1s'], lambda2=sched['lambda2'],
                                           persist=False).as_dict()
         del tc
         torch.cuda.empty_cache()
         log('tiles: %s' % json.dumps(tiles))
-    log('time course: %d pairs, %d combinations, sum I*J = %.3e, %d iterations' %
-        (len(pair_ij), n_combo, sum_ij, n_iter))
-    t0 = time.perf_counter()
-    res = batch.solve_time_course(coords, growth, graph_iters=50, group_size=args.group_size, **GRID, **sched)
+    log('time course: %d pairs, %d combinations, sum I*J = %.3e, %d iterations, %d per step' %
+        (len(pair_ij), n_combo, sum_ij, n_iter, step_iters))
+    if args.warmup:
+        tc = batch.prepare_time_course(dcoords, dgrowth)
+        with batch.TimeCourseRun(tc, eps_a, l1_a, l2_a, **run_kw) as run:
+            for w in range(args.warmup):
+                run.advance(step_iters)
+                torch.cuda.synchronize()
+                log('warmup %d' % w)
+        del tc
+        torch.cuda.empty_cache()
     torch.cuda.synchronize()
+    plan_buf = torch.empty(max(pair_ij), dtype=torch.float64, device='cuda')
+    sweep_ms, sweeps, exact_rows, shared = 0.0, 0, 0, None
+    res = tc = run = None
+    courses = 0
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        if run is None:
+            tc = batch.prepare_time_course(dcoords, dgrowth)
+            run = batch.TimeCourseRun(tc, eps_a, l1_a, l2_a, **run_kw)
+        left = run.advance(step_iters)
+        if left == 0:
+            res = run.result()
+            for p in range(len(pair_ij)):
+                n_i, n_j = res.shapes[p]
+                for e in range(n_combo):
+                    res.plan(p, e, out=plan_buf[:n_i * n_j].view(n_i, n_j))
+            run.close()
+            run = None
+            courses += 1
+            sweep_ms += res.stats['sweep_ms']
+            sweeps += res.stats['sweeps']
+            exact_rows += res.stats['exact_rows']
+            shared = res.stats['shared_group']
+        torch.cuda.synchronize()
+        log('step %d done' % s)
     elapsed = time.perf_counter() - t0
-    n_solves = len(pair_ij) * n_combo
-    sweep_s = res.stats['sweep_ms'] / 1000.0
-    avg_sweep = sweep_s / (2 * n_iter)  # per half-iteration over the whole grid (all groups)
+    if run is not None:  # a course left unfinished by the step count
+        part = run.result()
+        sweep_ms += part.stats['sweep_ms']
+        sweeps += part.stats['sweeps']
+        exact_rows += part.stats['exact_rows']
+        shared = part.stats['shared_group']
+        run.close()
+        run = None
+    done_fraction = args.steps * step_iters / float(n_iter)
+    value = n_solves * done_fraction / elapsed
+    avg_sweep = (sweep_ms / 1000.0) / max(sweeps, 1)  # per half-iteration over the whole grid
     # algorithmic work of one sweep over the whole batch: 4 bytes per cost entry
     # (one read shared by all combinations) and one exp per (entry, combination)
     alg_bytes = 4.0 * sum_ij
+    achieved = alg_bytes / avg_sweep / 1e9
+    rates = valu_rates(torch)
+    entries_per_s = sum_ij / avg_sweep
     out = {
-        'metric': 'transport maps/sec and Sinkhorn iters/sec at \u22641e-5 rel err, fused vs plain GPU',
-        'value': n_solves / elapsed, 'unit': 'maps/s', 'n_gpus': 1, 'steps': 1, 'warmup': int(bool(args.warmup)),
-        'ms_per_step': 1000.0 * elapsed, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'f64/f32', 'data': 'synthetic (seeded drifting Gaussian mixture, 39 days)',
-        'config': {'workload': '38-pair time course x eps{0.01,0.05,0.1} x lambda1{0.5,1,5}, fixed_iters',
+        'metric': 'transport maps/sec and Sinkhorn iters/sec at ≤1e-5 rel err, fused vs plain GPU',
+        'value': value, 'unit': 'maps/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': 1000.0 * elapsed / args.steps, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64/f32',
+        'data': 'synthetic (seeded drifting Gaussian mixture, 39 days, 30-d PCA coordinates resident in HBM)',
+        'config': {'workload': '38-pair time course x eps{0.01,0.05,0.1} x lambda1{0.5,1,5} = 342 fixed_iters '
+                               'solves, one step = %d iterations of every solve, %d steps = one course' %
+                               (step_iters, steps_per_course),
                    'cells_total': int(sum(counts)), 'sum_IJ': sum_ij, 'solves': n_solves, **sched,
-                   'tiles': tiles},
-        'sinkhorn_iters_per_s': n_solves * n_iter / elapsed,
-        'sweep_loop_s': sweep_s,
-        'shared_exponential_group': res.stats.get('shared_group'),
-        'rows_recomputed_exactly': res.stats.get('exact_rows'),
-        'roofline': {'bound': 'valu/exp', 'kernel': _tc_kernel_name(res.stats, tiles), 'avg_launch_us': avg_sweep * 1e6,
-                     'hbm_achieved': alg_bytes / avg_sweep / 1e9, 'hbm_peak': HBM_PEAK_GBS,
-                     'exp_per_s': n_combo * sum_ij / avg_sweep, 'unit': 'GB/s',
-                     'achieved': alg_bytes / avg_sweep / 1e9, 'peak': HBM_PEAK_GBS,
-                     'frac': alg_bytes / avg_sweep / 1e9 / HBM_PEAK_GBS, 'traffic': None},
+                   'step_iterations': step_iters, 'courses_completed': courses, 'tiles': tiles},
+        'timed_region': 'per course: costs + medians on the device, the full schedule, potentials, every FP64 map '
+                        'written once in HBM; no host copies (the 342 FP64 maps total %.0f GB; a D2H of them is '
+                        'not timed)' % (8.0 * n_combo * sum_ij / 1e9),
+        'sinkhorn_iters_per_s': n_solves * args.steps * step_iters / elapsed,
+        'sweep_loop_s_per_course': (sweep_ms / 1000.0) / max(courses, 1) if courses else None,
+        'shared_exponential_group': shared,
+        'rows_recomputed_exactly': exact_rows,
+        'roofline': {'bound': 'hbm', 'kernel': _tc_kernel_name({'shared_group': shared}, tiles),
+                     'avg_launch_us': avg_sweep * 1e6, 'alg_bytes_per_launch': alg_bytes,
+                     'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': _traffic('batch_shared_lds_kernel_timecourse_38pairs_9combos'),
+                     'traffic_source': 'profiles/batch_shared_lds_kernel_timecourse_38pairs_9combos.traffic.json '
+                                       '(calibrated FETCH_SIZE + WRITE_SIZE, not measured in this run)',
+                     'compute': {'note': 'the sweep is VALU-issue-bound, not HBM-bound: one FP32 cost read '
+                                         'serves 9 combinations',
+                                 'entries_per_s': entries_per_s,
+                                 'entry_combinations_per_s': entries_per_s * n_combo,
+                                 'exp_f32_per_s': entries_per_s * len(GRID['epsilons']),
+                                 'measured_exp_f32_peak_per_s': rates['exp_f32'],
+                                 'exp_frac': entries_per_s * len(GRID['epsilons']) / rates['exp_f32'],
+                                 'measured_rates_per_s': rates}},
     }
     if args.no_parity:
         return out
-    # parity: a few (pair, combination) maps against the single-pair solver
-    # (itself checked against the FP64 oracle) on the same device cost
-    worst = 0.0
-    for p in (0, int(np.argmax(pair_ij)), len(pair_ij) - 1):
-        cost = ot.DeviceCost.from_coordinates(coords[p], coords[p + 1])
-        for e in (0, 4, 8):
-            eps, lam1 = res.combos[e]
-            R1 = ot.transport_stablev2(cost, G=growth[p], device_out=True, epsilon=eps, lambda1=lam1, **sched)
-            R2 = res.plan(p, e, device_out=True)
-            worst = max(worst, float(torch.linalg.norm(R1 - R2) / torch.linalg.norm(R1)))
-            del R1, R2
-    out['parity_vs_single_solver_rel_frob'] = worst
+    if not args.no_baselines:
+        out.update(_timecourse_baselines(torch, coords, growth, pair_ij, n_combo, sched, n_iter, 1.0 / value * n_solves,
+                                         res))
     order = np.argsort(pair_ij)
     out['cost_split_bf16'] = [cost_gemm_report(torch, coords[p], coords[p + 1])
-                          for p in (int(order[0]), int(order[len(order) // 2]), int(order[-1]))]
-    if not args.no_baselines:
-        out.update(_timecourse_baselines(torch, coords, growth, pair_ij, n_combo, sched, n_iter, elapsed))
+                              for p in (int(order[0]), int(order[len(order) // 2]), int(order[-1]))]
+    out['secondary_config_1'] = _secondary_pair(torch)
     return out
+
+
+def _secondary_pair(torch):
+    """One full map of the 15k x 15k pair (growth_iters=3, 6-term split-BF16
+    cost) timed once, with its marginal identity: the single-pair path."""
+    import wot_amd.ot as ot
+    from wot_amd.ot import device as dev
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(CFG['n_source'], CFG['n_target'], dim=CFG['dim'], n_components=CFG['n_components'],
+                  seed=CFG['seed'])
+    x = torch.from_numpy(dp.x).cuda()
+    y = torch.from_numpy(dp.y).cuda()
+    g = torch.from_numpy(dp.growth).cuda()
+    one_map(ot, dev, x, y, g)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cost, res, sweep_ms, sweeps = one_map(ot, dev, x, y, g)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    q = float(res['G_last'].mean())
+    want = q * CFG['n_source'] * torch.exp(-res['g'] / SOLVER['lambda2'])
+    colsum = res['plan'].sum(0) * CFG['n_target']
+    marg = float((colsum - want).abs().max() / want.abs().max())
+    avg = sweep_ms / 1000.0 / sweeps
+    alg = 4.0 * CFG['n_source'] * CFG['n_target']
+    return {'workload': 'one day pair 15000x15000, d=30, fixed_iters, growth_iters=3, 6-term split-BF16 cost',
+            'maps_per_s': 1.0 / el, 'ms_per_map': 1000.0 * el, 'marginal_identity_rel_err': marg,
+            'roofline': {'bound': 'hbm', 'kernel': 'lse_rows_kernel', 'avg_launch_us': avg * 1e6,
+                         'achieved': alg / avg / 1e9, 'peak': HBM_PEAK_GBS, 'frac': alg / avg / 1e9 / HBM_PEAK_GBS}}
 
 
 def _parse_tile(text):
@@ -305,12 +430,21 @@ def _fit_and_sum(samples, pair_ij, n_combo):
     return n_combo * sum(a + b * ij for ij in pair_ij), a, b
 
 
-def _timecourse_baselines(torch, coords, growth, pair_ij, n_combo, sched, n_iter, elapsed):
+def _timecourse_baselines(torch, coords, growth, pair_ij, n_combo, sched, n_iter, course_s, res):
+    """Plain GPU: the torch FP64 restatement solves the smallest, median and
+    largest pairs in full (eps=0.05, lambda1=1; the solve time does not depend
+    on the grid point), fitted as t = a + b*I*J and summed over all 342 solves;
+    its three plans also give the parity of the batched maps of the same
+    (pair, combination).  CPU: the FP64 numpy oracle solves the smallest pair
+    in full and times 20 iterations + one kernel build on the median and
+    largest pairs; iteration and build times are fitted linearly in I*J and
+    summed over every solve's 4000 iterations and 61 builds."""
     from wot_amd import plain_gpu
     from wot_amd.ot import device as dev
     order = np.argsort(pair_ij)
     picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
-    samples = []
+    e_ref = res.combos.index((0.05, 1.0)) if res is not None else None
+    samples, parity = [], []
     for p in picks:
         X = torch.from_numpy(coords[p]).cuda()
         Y = torch.from_numpy(coords[p + 1]).cuda()
@@ -324,45 +458,75 @@ def _timecourse_baselines(torch, coords, growth, pair_ij, n_combo, sched, n_iter
                                      **dict(sched, scaling_iter=3, extra_iter=0))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        plain_gpu.transport_stablev2(C, G=growth[p], epsilon=0.05, lambda1=1.0, **sched)
+        R0 = plain_gpu.transport_stablev2(C, G=growth[p], epsilon=0.05, lambda1=1.0, **sched)
         torch.cuda.synchronize()
         samples.append((pair_ij[p], time.perf_counter() - t0))
-        del C
+        if res is not None:
+            R1 = res.plan(p, e_ref, device_out=True)
+            rel = float(torch.linalg.norm(R1 - R0) / torch.linalg.norm(R0))
+            marg = max(float((R1.sum(a) - R0.sum(a)).abs().max() / R0.sum(a).abs().max()) for a in (0, 1))
+            parity.append({'pair': p, 'shape': list(R0.shape), 'rel_frob': rel, 'marginal_rel': marg})
+            del R1
+        del C, R0
         torch.cuda.empty_cache()
         log('plain GPU sample pair %d (I*J=%d): %.2f s' % (p, pair_ij[p], samples[-1][1]))
     plain_total, a, b = _fit_and_sum(samples, pair_ij, n_combo)
-    # CPU: the FP64 oracle on the smallest pair, 50 + 50 iterations, scaled by I*J
     from oracle import ot_oracle as O
+    lam2, eps0, inner = sched['lambda2'], sched['epsilon0'], sched['inner_iter_max']
+    n_build = 1 + sched['scaling_iter'] // inner
+    cpu_pts = []
+    t_start = time.perf_counter()
     p = picks[0]
     C = O.default_cost_matrix(coords[p], coords[p + 1])
     t0 = time.perf_counter()
-    O.transport_stablev2(C, 1.0, SOLVER['lambda2'], 0.05, 50, growth[p], 1e4, 1.0, 0, 50)
-    ta = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.transport_stablev2(C, 1.0, SOLVER['lambda2'], 0.05, 50, growth[p], 1e4, 1.0, 50, 50)
-    tb = time.perf_counter() - t0
-    t_it = max((tb - ta) / 50.0, 0.0)
-    t_build = max((ta - 50 * t_it) / 2.0, 0.0)
-    n_build = 1 + sched['scaling_iter'] // sched['inner_iter_max']
-    per_ij = (n_build * t_build + n_iter * t_it) / pair_ij[p]
-    cpu_total = n_combo * per_ij * sum(pair_ij)
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([i.get('num_threads', 1) for i in threadpool_info()] + [1])
-    except Exception:  # pragma: no cover
-        threads = os.cpu_count()
+    O.transport_stablev2(C, 1.0, lam2, 0.05, sched['scaling_iter'], growth[p], 1e4, eps0, sched['extra_iter'], inner)
+    t_full_small = time.perf_counter() - t0
+    for p in picks[1:]:
+        C = O.default_cost_matrix(coords[p], coords[p + 1])
+        t0 = time.perf_counter()
+        O.transport_stablev2(C, 1.0, lam2, 0.05, 1, growth[p], 1e4, eps0, 0, inner)   # 1 build + 1 iteration
+        ta = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.transport_stablev2(C, 1.0, lam2, 0.05, 1, growth[p], 1e4, eps0, 20, inner)  # 1 build + 21 iterations
+        tb = time.perf_counter() - t0
+        t_it = max((tb - ta) / 20.0, 0.0)
+        cpu_pts.append((pair_ij[p], t_it, max(ta - t_it, 0.0)))
+        del C
+    sample_wall = time.perf_counter() - t_start
+    # per-solve time of pair p: the small pair measured in full; the others from
+    # the linear fits of iteration and build time in I*J through the two points
+    (x1, i1, b1), (x2, i2, b2) = cpu_pts
+    def lin(x, y1, y2):
+        return y1 + (y2 - y1) * (x - x1) / max(x2 - x1, 1)
+    per_solve = []
+    for k, ij in enumerate(pair_ij):
+        if k == picks[0]:
+            per_solve.append(t_full_small)
+        else:
+            per_solve.append(n_iter * max(lin(ij, i1, i2), 0.0) + n_build * max(lin(ij, b1, b2), 0.0))
+    cpu_total = n_combo * sum(per_solve)
+    hi = host_info()
     n_solves = len(pair_ij) * n_combo
-    return {
+    out = {
         'plain_gpu': {'value': n_solves / plain_total, 'unit': 'maps/s',
                       'sample': 'torch FP64 dense-K restatement: full solves of the smallest, median and largest '
                                 'pairs timed, t = a + b*I*J fitted and summed over all 342 solves',
                       'fit_a_s': a, 'fit_b_s': b, 'samples': samples},
-        'speedup_vs_plain_gpu': plain_total / elapsed,
-        'cpu_baseline': {'value': n_solves / cpu_total, 'unit': 'maps/s', 'cores': threads, 'kind': 'port',
-                         'sample': 'FP64 numpy oracle on the smallest pair (50 scheduled + 50 extra iterations '
-                                   'timed apart), scaled by I*J to every solve'},
-        'speedup_vs_cpu': cpu_total / elapsed,
+        'speedup_vs_plain_gpu': plain_total / course_s,
+        'cpu_baseline': {'value': n_solves / cpu_total, 'unit': 'maps/s', 'cores': hi['blas_threads'],
+                         'kind': 'port', 'nproc': hi['nproc'], 'cpu_model': hi['cpu_model'],
+                         'sample': 'FP64 numpy oracle: the smallest pair solved in full (%.1f s); 20 iterations and '
+                                   'one kernel build timed on the median and largest pairs, fitted linearly in I*J '
+                                   'and summed over 342 solves x (4000 iterations + %d builds); sample %.1f s' %
+                                   (t_full_small, n_build, sample_wall),
+                         'points_ij_iter_build_s': cpu_pts},
+        'speedup_vs_cpu': cpu_total / course_s,
     }
+    if parity:
+        out['parity_vs_plain_gpu'] = {'combination': [0.05, 1.0], 'pairs': parity,
+                                      'max_rel_frob': max(q['rel_frob'] for q in parity),
+                                      'max_marginal_rel': max(q['marginal_rel'] for q in parity)}
+    return out
 
 
 def bench_adjoint(args):
@@ -1003,15 +1167,15 @@ def bench_validation(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=3)
-    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-baselines', action='store_true', help='skip the plain-GPU and CPU samples')
     ap.add_argument('--large-n', type=int, default=60000, help='large only: cells per day')
     ap.add_argument('--group-size', type=int, default=None,
                     help='timecourse only: combinations per batched run (default: the whole grid)')
-    ap.add_argument('--config', default='pair', choices=['pair', 'gap', 'timecourse', 'adjoint', 'large', 'tmap', 'pca', 'validation'],
-                    help='pair: one 15k x 15k map (default); gap: the same pair with the duality-gap solver; '
-                         'timecourse: 38 pairs x 9-point grid; '
+    ap.add_argument('--config', default='timecourse', choices=['pair', 'gap', 'timecourse', 'adjoint', 'large', 'tmap', 'pca', 'validation'],
+                    help='timecourse: 38 pairs x 9-point grid (default); pair: one 15k x 15k map; '
+                         'gap: the same pair with the duality-gap solver; '
                          'adjoint: gradient of the converged objective, 8k x 8k; '
                          'tmap: trajectories through the 38 maps of the time course; '
                          'pca: local PCA of one 15k + 15k cell day pair; '

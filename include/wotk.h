@@ -182,6 +182,36 @@ int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pairs, int32_t
                         double *g_out, int32_t graph_iters, int32_t flags, void *workspace,
                         size_t workspace_bytes, wotk_stats *stats, void *stream);
 
+/* Stepwise form of the same batched solve, with per-pair parameters (the
+ * per-pair epsilon / lambda1 / lambda2 of a day-pair configuration,
+ * wot/ot/initializer.py:135-183, merged at ot_model.py:231, 238).
+ * wotk_batch_begin sets up the run in `workspace` (sized by
+ * wotk_batch_workspace_size) and captures the graph; epsilon, lambda1 and
+ * lambda2 are HOST arrays [pair][combination].  growth_per_combo = 0: G holds
+ * one growth vector per pair (at g_offset); 1: one per (pair, combination),
+ * [combination][I_p] starting at g_offset (the learned growth of the
+ * previous growth iteration, optimal_transport.py:26-28).
+ * wotk_batch_advance runs the next n_iterations of the schedule (3000 + 1000
+ * for the reference defaults; clamped to what is left) on `stream` and
+ * reports the iterations left.  wotk_batch_potentials copies the current
+ * potentials ([pair][combination][I_p] / [J_p], device), the final epsilon of
+ * every (pair, combination) (HOST, may be NULL) and the stats (sweep_ms over
+ * every advance so far); it synchronises `stream` when stats is given.
+ * wotk_batch_end releases the run's host objects (graph, events, streams);
+ * the caller keeps `workspace` alive until then.  A run is driven from one
+ * host thread at a time. */
+typedef struct wotk_batch_run wotk_batch_run;
+int wotk_batch_begin(const float *cost, const wotk_batch_pair *pairs, int32_t n_pairs, const double *G,
+                     int32_t growth_per_combo, int32_t n_combos, const double *epsilon,
+                     const double *lambda1, const double *lambda2, int64_t scaling_iter,
+                     double epsilon0, int64_t extra_iter, int64_t inner_iter_max,
+                     int32_t graph_iters, int32_t flags, void *workspace, size_t workspace_bytes,
+                     wotk_batch_run **run, void *stream);
+int wotk_batch_advance(wotk_batch_run *run, int64_t n_iterations, int64_t *remaining, void *stream);
+int wotk_batch_potentials(wotk_batch_run *run, double *f_out, double *g_out, double *eps_final,
+                          wotk_stats *stats, void *stream);
+void wotk_batch_end(wotk_batch_run *run);
+
 /* ---------------------------------------------------------------------------
  * transport_stablev2 (optimal_transport.py:167-236) on the cost of
  * ot_model.py:242-253 computed from the coordinates X (I x d), Y (J x d) and
@@ -377,6 +407,20 @@ int32_t wotk_num_buckets(int32_t kernel);
  * default; a tile that is not a compiled variant is rejected (WOTK_EINVAL) */
 int wotk_set_tile(int32_t kernel, int32_t bucket, const wotk_tile *tile);
 int wotk_get_tile(int32_t kernel, int32_t bucket, wotk_tile *out);
+
+/* ---------------------------------------------------------------------------
+ * Instruction-rate probe (measurement only; no counterpart in the reference):
+ * the issue rate of the instructions the compute-bound batched sweep is built
+ * from, for its roofline.  rates_host (HOST, >= 7 entries) receives, in
+ * operations per second over the whole device: [0] v_exp_f32, [1] FP64 FMA,
+ * [2] FP32 FMA, [3] FP64<->FP32 converts, [4] FP64 round-to-nearest (+ add),
+ * [5] v_ldexp_f32, [6] the sweep's exact split exponential of an FP64
+ * exponent (one whole entry: 2 FP64 FMAs, clamp, rint, subtract, two
+ * converts, exp2, ldexp).  sink: device scratch of wotk_probe_sink_bytes()
+ * bytes (never written in practice).
+ * ------------------------------------------------------------------------- */
+size_t wotk_probe_sink_bytes(void);
+int wotk_probe_valu_rates(double *rates_host, int32_t n_rates, void *sink, size_t sink_bytes, void *stream);
 
 #ifdef __cplusplus
 }

@@ -90,6 +90,11 @@ SIGNATURES = {
     'wotk_batch_workspace_size': (_SZ, [_P, _I32, _I32, _I64]),
     'wotk_batch_stablev2': (ctypes.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _D, _I64, _D, _D, _I64, _I64, _P, _P,
                                            _I32, _I32, _P, _SZ, ctypes.POINTER(Stats), _P]),
+    'wotk_batch_begin': (ctypes.c_int, [_P, _P, _I32, _P, _I32, _I32, _P, _P, _P, _I64, _D, _I64, _I64, _I32, _I32,
+                                        _P, _SZ, ctypes.POINTER(ctypes.c_void_p), _P]),
+    'wotk_batch_advance': (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_int64), _P]),
+    'wotk_batch_potentials': (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(Stats), _P]),
+    'wotk_batch_end': (None, [_P]),
     'wotk_fused_workspace_size': (_SZ, [_I64, _I64, _I64]),
     'wotk_transport_stablev2_coords': (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64, _D, _D, _I64,
                                                       _I64, _I32, _P, _P, _P, _P, _P, _SZ, ctypes.POINTER(Stats),
@@ -112,6 +117,8 @@ SIGNATURES = {
     'wotk_emd': (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _I64, ctypes.POINTER(ctypes.c_double), _P,
                                 ctypes.POINTER(_I64), ctypes.POINTER(_I32)]),
     'wotk_pca_center': (ctypes.c_int, [_P, _I32, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _SZ, _P]),
+    'wotk_probe_sink_bytes': (_SZ, []),
+    'wotk_probe_valu_rates': (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), _I32, _P, _SZ, _P]),
     'wotk_tile_variants': (_I32, [_I32, ctypes.POINTER(Tile), _I32]),
     'wotk_shape_bucket': (_I32, [_I32, _I64, _I64, _I32]),
     'wotk_num_buckets': (_I32, [_I32]),

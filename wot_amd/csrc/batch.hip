@@ -21,6 +21,8 @@
 // online-maximum recomputation for any row whose sum under- or overflows.
 #include <algorithm>
 #include <array>
+#include <cmath>
+#include <memory>
 #include <utility>
 #include <vector>
 
@@ -37,11 +39,11 @@ struct PairDev {
     long long pot_off;    // offset of this side's potentials ([E][pot_ld]) in the pot buffer
     long long other_off;  // offset of the other side's potentials ([E][other_ld])
     int pot_ld, other_ld; // row counts padded to a multiple of 4
-    long long logw_off;   // offset of log weights ([nrows]) or -1 for the pair's scalar
+    long long logw_off;   // offset of the pair's log weights ([E][nrows]) or -1 for its log q
     int pair;
 };
 
-struct IterParam {  // one (iteration, combination)
+struct IterParam {  // one (iteration, pair, combination)
     double k;       // log2(e) / eps
     double eps;
     double alpha1, alpha2;
@@ -57,14 +59,15 @@ struct BatchArgs {
     const PairDev *pairs;
     const Tile *tiles;
     int ntiles;
-    const IterParam *table;    // [iteration][E]
+    const IterParam *table;    // [iteration][pair][E]
     const int *iter;           // device iteration counter
     int E;
+    int n_pairs;
     int side;                  // 0: f from g over C (alpha1), 1: g from f over C^T (alpha2)
     double *pot;               // this side's potentials
     const double *other;       // the other side's potentials
-    const double *logw;        // per-row log weights (source side)
-    const double *logq;        // per-pair log q (target side)
+    const double *logw;        // log weights of the source rows ([pair][E][rows])
+    const double *logq;        // log q of the target side ([pair][E])
     const double *log_n_other; // per side-pair log(#columns)
     int *queue;                // work-queue counter
     float *fac;                // shared kernel: this side's column factors ([E][pot_ld], like pot)
@@ -115,6 +118,11 @@ __device__ __forceinline__ float factor_of(double kn, double pot_e, double pot_r
 }
 
 __device__ __forceinline__ float bexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// the E combinations' parameters of pair `pair` at iteration `it`
+__device__ __forceinline__ const IterParam *params_at(const BatchArgs &ba, int it, int pair) {
+    return ba.table + ((size_t)it * ba.n_pairs + pair) * ba.E;
+}
 
 // 2^y for an FP64 exponent y, exact up to the FP32 exp2 of a fraction: y = n + r
 // with n = rint(y) and |r| <= 1/2, so the rounding of the exponent to FP32 costs
@@ -226,9 +234,10 @@ __device__ __forceinline__ void lane_sums(const float *s, double (*red_a)[NW]) {
 
 // new potential of (row, combination) from l2 = log2 sum_j 2^((h_j - C_ij) k)
 __device__ __forceinline__ double new_potential(const BatchArgs &ba, const PairDev &pd, const IterParam &ip, int row,
-                                                int side_pair, double l2) {
+                                                int side_pair, double l2, int e) {
     const double lse = l2 * kLn2;  // ln sum_j exp((h_j - C_ij)/eps)
-    const double lw = (pd.logw_off >= 0) ? ba.logw[pd.logw_off + row] : ba.logq[pd.pair];
+    const double lw = (pd.logw_off >= 0) ? ba.logw[pd.logw_off + (long long)e * pd.nrows + row]
+                                         : ba.logq[(long long)pd.pair * ba.E + e];
     const double alpha = ba.side == 0 ? ip.alpha1 : ip.alpha2;
     return alpha * ip.eps * (lw + ba.log_n_other[side_pair] - lse);
 }
@@ -244,10 +253,6 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
     __shared__ double l2_sh[RE];  // log2 sum_j 2^((h_j - C_ij) k) per (row, combination)
     const int tid = threadIdx.x;
     const int it = *ba.iter;
-    const IterParam *prm = ba.table + (size_t)it * E;
-    double kk[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) kk[e] = prm[e].k;
 
     // Every iteration ends on a barrier: lane-specific work sits right after the
     // barrier at the top, never before the back-edge (where the compiler may
@@ -261,6 +266,10 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
         if (t >= ba.ntiles) break;
         const Tile tl = ba.tiles[t];
         const PairDev pd = ba.pairs[tl.side_pair];
+        const IterParam *prm = params_at(ba, it, pd.pair);
+        double kk[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) kk[e] = prm[e].k;
         const float *A = ba.cost + pd.a_off;
         const double *h = ba.other + pd.other_off;
         double *pot = ba.pot + pd.pot_off;
@@ -322,7 +331,7 @@ __global__ __launch_bounds__(THREADS) void batch_sweep_kernel(BatchArgs ba) {
             const int m = bad_row[r];
             if (row < pd.nrows) {
                 if (!((m >> e) & 1))
-                    pot[(long long)e * pd.pot_ld + row] = new_potential(ba, pd, prm[e], row, tl.side_pair, l2_sh[tid]);
+                    pot[(long long)e * pd.pot_ld + row] = new_potential(ba, pd, prm[e], row, tl.side_pair, l2_sh[tid], e);
                 if (m && e == 0) queue_row(ba, tl.side_pair, row, m);
             }
         }
@@ -365,11 +374,6 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
     __shared__ double pot_sh[RE];
     const int tid = threadIdx.x;
     const int it = *ba.iter;
-    const IterParam *prm = ba.table + (size_t)it * E;
-    const IterParam *nxt = ba.table + (size_t)(it + ba.k_next) * E;  // k the factors are read with
-    double kg[NG];
-#pragma unroll
-    for (int g = 0; g < NG; ++g) kg[g] = prm[g * NL + L0].k;
     if (tid < E) fmax_sh[tid] = 0.f;
     int prev_pair = -1;  // pair of the previous tile, whose factor maxima are flushed at the top
 
@@ -387,6 +391,11 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
         const Tile tl = ba.tiles[t];
         const PairDev pd = ba.pairs[tl.side_pair];
         prev_pair = pd.pair;
+        const IterParam *prm = params_at(ba, it, pd.pair);
+        const IterParam *nxt = params_at(ba, it + ba.k_next, pd.pair);  // k the factors are read with
+        double kg[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) kg[g] = prm[g * NL + L0].k;
         const float *A = ba.cost + pd.a_off;
         const double *h = ba.other + pd.other_off;
         const float *B = ba.fac_other + pd.other_off;
@@ -458,7 +467,7 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
             const int r = tid / E, e = tid % E;
             const int row = tl.row0 + r;
             const int m = widen_to_groups<NG, NL>(bad_row[r]);
-            const double p = new_potential(ba, pd, prm[e], min(row, pd.nrows - 1), tl.side_pair, l2_sh[tid]);
+            const double p = new_potential(ba, pd, prm[e], min(row, pd.nrows - 1), tl.side_pair, l2_sh[tid], e);
             pot_sh[tid] = p;
             if (row < pd.nrows) {
                 if (!((m >> e) & 1)) pot[(long long)e * pd.pot_ld + row] = p;
@@ -512,11 +521,6 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 
     __shared__ double pot_sh[NWAVES][RE];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int it = *ba.iter;
-    const IterParam *prm = ba.table + (size_t)it * E;
-    const IterParam *nxt = ba.table + (size_t)(it + ba.k_next) * E;
-    double kg[NG];
-#pragma unroll
-    for (int g = 0; g < NG; ++g) kg[g] = prm[g * NL + L0].k;
     if (tid < E) fmax_sh[tid] = 0.f;
     int prev_pair = -1;  // pair of the previous tile, whose factor maxima are flushed at the top
 
@@ -534,6 +538,11 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 
         const Tile tl = ba.tiles[t];
         const PairDev pd = ba.pairs[tl.side_pair];
         prev_pair = pd.pair;
+        const IterParam *prm = params_at(ba, it, pd.pair);
+        const IterParam *nxt = params_at(ba, it + ba.k_next, pd.pair);
+        double kg[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) kg[g] = prm[g * NL + L0].k;
         const float *A = ba.cost + pd.a_off;
         const double *h = ba.other + pd.other_off;
         const float *B = ba.fac_other + pd.other_off;
@@ -661,7 +670,7 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 
         const bool row_ok = mine && !((m >> my_e) & 1);
         if (lane < RE) {
             const double p =
-                new_potential(ba, pd, prm[my_e], min(my_row, pd.nrows - 1), tl.side_pair, log2(S) - fkl);
+                new_potential(ba, pd, prm[my_e], min(my_row, pd.nrows - 1), tl.side_pair, log2(S) - fkl, my_e);
             pot_sh[wave][lane] = p;
             if (row_ok) pot[(long long)my_e * pd.pot_ld + my_row] = p;
             if (mine && m && my_e == 0) queue_row(ba, tl.side_pair, my_row, m);
@@ -699,13 +708,13 @@ __global__ __launch_bounds__(THREADS) void batch_repair_kernel(BatchArgs ba) {
     const int nb = __builtin_amdgcn_readfirstlane(*ba.n_bad);
     if ((int)blockIdx.x >= nb) return;
     const int it = *ba.iter;
-    const IterParam *prm = ba.table + (size_t)it * E;
-    const IterParam *nxt = ba.table + (size_t)(it + ba.k_next) * E;
     if (blockIdx.x == 0 && tid == 0) atomicAdd(ba.exact_rows, (unsigned long long)nb);
     for (int b = blockIdx.x; b < nb; b += gridDim.x) {
         const int sp = ba.bad_list[3 * b], row = ba.bad_list[3 * b + 1];
         const int mask = __builtin_amdgcn_readfirstlane(ba.bad_list[3 * b + 2]);
         const PairDev pd = ba.pairs[sp];
+        const IterParam *prm = params_at(ba, it, pd.pair);
+        const IterParam *nxt = params_at(ba, it + ba.k_next, pd.pair);
         const float *A = ba.cost + pd.a_off;
         const double *h = ba.other + pd.other_off;
         double *pot = ba.pot + pd.pot_off;
@@ -715,7 +724,7 @@ __global__ __launch_bounds__(THREADS) void batch_repair_kernel(BatchArgs ba) {
         if (tid < E) {
             const long long at = (long long)tid * pd.pot_ld + row;
             const bool redo = (mask >> tid) & 1;
-            const double p = redo ? new_potential(ba, pd, prm[tid], row, sp, l2_sh[tid]) : pot[at];
+            const double p = redo ? new_potential(ba, pd, prm[tid], row, sp, l2_sh[tid], tid) : pot[at];
             pot_sh[tid] = p;
             if (redo) pot[at] = p;
         }
@@ -757,14 +766,18 @@ __global__ void batch_fill_kernel(double *x, long long n, double v) {
         x[i] = v;
 }
 
-// log weights of the source rows and log q = log(mean G) per pair
+// log weights of the source rows and log q = log(mean G) per (pair,
+// combination): block (p, e); the growth of combination e starts at
+// G + g_off[p] + e * n_p when every combination has its own (per_combo), else
+// all share G + g_off[p]
 __global__ __launch_bounds__(256) void batch_growth_kernel(const double *__restrict__ G, const long long *__restrict__ g_off,
                                                            const int *__restrict__ n_src, double *__restrict__ logw,
-                                                           const long long *__restrict__ lw_off, double *__restrict__ logq) {
-    const int p = blockIdx.x;
-    const double *g = G + g_off[p];
-    double *lw = logw + lw_off[p];
+                                                           const long long *__restrict__ lw_off, double *__restrict__ logq,
+                                                           int E, int per_combo) {
+    const int p = blockIdx.x, e = blockIdx.y;
     const int n = n_src[p];
+    const double *g = G + g_off[p] + (per_combo ? (long long)e * n : 0);
+    double *lw = logw + lw_off[p] + (long long)e * n;
     __shared__ double red[4];
     double acc = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -774,7 +787,7 @@ __global__ __launch_bounds__(256) void batch_growth_kernel(const double *__restr
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) logq[p] = log((red[0] + red[1] + red[2] + red[3]) / n);
+    if (threadIdx.x == 0) logq[(long long)p * E + e] = log((red[0] + red[1] + red[2] + red[3]) / n);
 }
 
 // -inf on the padding rows [n, n_pad) of every (pair, combination) block:
@@ -833,11 +846,11 @@ static size_t batch_carve(Arena &ar, const wotk_batch_pair *pairs, int n_pairs, 
     w->g = ar.take<double>((size_t)ng * E);
     w->ff = ar.take<float>((size_t)nf * E);
     w->fg = ar.take<float>((size_t)ng * E);
-    w->logw = ar.take<double>(nlw);
-    w->logq = ar.take<double>(n_pairs);
+    w->logw = ar.take<double>((size_t)nlw * E);
+    w->logq = ar.take<double>((size_t)n_pairs * E);
     w->lognf = ar.take<double>(n_pairs);
     w->logng = ar.take<double>(n_pairs);
-    w->table = ar.take<IterParam>((size_t)(n_iter + 1) * E);
+    w->table = ar.take<IterParam>((size_t)(n_iter + 1) * n_pairs * E);
     w->pairs_f = ar.take<PairDev>(n_pairs);
     w->pairs_g = ar.take<PairDev>(n_pairs);
     w->tiles_f = ar.take<Tile>(tf);
@@ -992,29 +1005,70 @@ static int persistent_grid(const BatchVariant &v) {
     return ncu * v.occupancy();
 }
 
-extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pairs, int32_t n_pairs, const double *G,
-                                   int32_t n_combos, const double *epsilon, const double *lambda1, double lambda2,
-                                   int64_t scaling_iter, double tau, double epsilon0, int64_t extra_iter,
-                                   int64_t inner_iter_max, double *f_out, double *g_out, int32_t graph_iters,
-                                   int32_t flags, void *workspace, size_t workspace_bytes, wotk_stats *stats,
-                                   void *stream) {
+// Host state of one batched run between wotk_batch_begin and wotk_batch_end:
+// the kernel arguments, the captured graph and the timing events.  Every HIP
+// object it holds is released by its destructor, whatever path ends the run.
+struct wotk_batch_run {
+    BatchArgs af{}, ag{};
+    SweepFn sweep = nullptr;
+    int grid_f = 0, grid_g = 0, grid_r = 0;
+    int E = 0, n_pairs = 0, group = 0, graph_iters = 0;
+    int64_t n_iter = 0, done = 0;
+    BatchWork w{};
+    std::vector<PairDev> pf, pg;
+    std::vector<int64_t> n_i, n_j;
+    std::vector<double> eps_final;  // [pair][E]
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cs = nullptr;  // capture stream
+    bool capturing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timing;
+
+    ~wotk_batch_run() {
+        if (capturing) {
+            hipGraph_t partial = nullptr;
+            if (hipStreamEndCapture(cs, &partial) == hipSuccess && partial) (void)hipGraphDestroy(partial);
+        }
+        for (auto &ev : timing) {
+            if (ev.first) (void)hipEventDestroy(ev.first);
+            if (ev.second) (void)hipEventDestroy(ev.second);
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cs) (void)hipStreamDestroy(cs);
+    }
+};
+
+extern "C" int wotk_batch_begin(const float *cost, const wotk_batch_pair *pairs, int32_t n_pairs, const double *G,
+                                int32_t growth_per_combo, int32_t n_combos, const double *epsilon,
+                                const double *lambda1, const double *lambda2, int64_t scaling_iter, double epsilon0,
+                                int64_t extra_iter, int64_t inner_iter_max, int32_t graph_iters, int32_t flags,
+                                void *workspace, size_t workspace_bytes, wotk_batch_run **run_out, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     const int E = n_combos;
-    WOTK_REQUIRE(n_pairs > 0 && pairs && cost && G, "empty batch");
+    WOTK_REQUIRE(run_out != nullptr, "run handle pointer is required");
+    *run_out = nullptr;
+    WOTK_REQUIRE(n_pairs > 0 && pairs && cost && G && epsilon && lambda1 && lambda2, "empty batch");
     WOTK_REQUIRE(E >= 1 && E <= kMaxCombos && pick(E), "unsupported number of combinations %d", E);
     WOTK_REQUIRE(scaling_iter >= 0 && extra_iter >= 0 && scaling_iter + extra_iter > 0 && inner_iter_max > 0,
                  "bad iteration counts");
-    (void)tau;  // absorptions are exact in the log domain (see sinkhorn.hip)
+    WOTK_REQUIRE(scaling_iter + extra_iter < (int64_t)1 << 30, "too many iterations");
     for (int p = 0; p < n_pairs; ++p) {
         const wotk_batch_pair &q = pairs[p];
         WOTK_REQUIRE(q.n_i > 0 && q.n_j > 0 && q.n_i < (1 << 30) && q.n_j < (1 << 30), "pair %d: bad shape", p);
         WOTK_REQUIRE(q.ldc % 4 == 0 && q.ldc >= round_up(q.n_j, 4) && q.ldct % 4 == 0 && q.ldct >= round_up(q.n_i, 4),
                      "pair %d: leading dimensions must be multiples of 4 covering the padded rows", p);
         WOTK_REQUIRE(q.c_offset % 4 == 0 && q.ct_offset % 4 == 0, "pair %d: offsets must be multiples of 4", p);
+        for (int e = 0; e < E; ++e) {
+            const double ep = epsilon[p * E + e], l1 = lambda1[p * E + e], l2 = lambda2[p * E + e];
+            WOTK_REQUIRE(ep > 0.0 && l1 > 0.0 && l2 > 0.0 && std::isfinite(ep) && std::isfinite(l1) && std::isfinite(l2),
+                         "pair %d, combination %d: epsilon and lambdas must be positive", p, e);
+        }
     }
     const int64_t n_iter = scaling_iter + extra_iter;
+    std::unique_ptr<wotk_batch_run> run(new wotk_batch_run());
     Arena ar(workspace, workspace_bytes);
-    BatchWork w;
+    BatchWork &w = run->w;
     int64_t nf, ng, ntf, ntg;
     batch_carve(ar, pairs, n_pairs, E, n_iter, &nf, &ng, &w, &ntf, &ntg);
     if (!workspace || !ar.ok) return fail(WOTK_EWORKSPACE, "batch workspace too small");
@@ -1025,13 +1079,15 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
         entries += 2.0 * (double)pairs[p].n_i * (double)pairs[p].n_j;
         rows_total += (double)pairs[p].n_i + (double)pairs[p].n_j;
     }
-    // shared exponentials: the largest group size of consecutive equal epsilons with a compiled shape
+    // shared exponentials: the largest group size of consecutive equal epsilons
+    // (in every pair) with a compiled shape
     int group = 0;
     if (flags & WOTK_BATCH_SHARE_EXP)
         for (int nl = E; nl >= 2 && !group; --nl) {
             if (E % nl || !pick_shared(E / nl, nl)) continue;
             bool equal = true;
-            for (int e = 0; e < E; ++e) equal = equal && epsilon[e] == epsilon[e - e % nl];
+            for (int p = 0; p < n_pairs; ++p)
+                for (int e = 0; e < E; ++e) equal = equal && epsilon[p * E + e] == epsilon[p * E + e - e % nl];
             if (equal) group = nl;
         }
     const int64_t mean_len = (int64_t)(entries / rows_total);
@@ -1048,7 +1104,9 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     }
 
     // host-side tables: pairs, tiles (longest rows first), iteration parameters
-    std::vector<PairDev> pf(n_pairs), pg(n_pairs);
+    std::vector<PairDev> &pf = run->pf, &pg = run->pg;
+    pf.resize(n_pairs);
+    pg.resize(n_pairs);
     std::vector<long long> goff(n_pairs), lwoff(n_pairs);
     std::vector<int> nsrc(n_pairs);
     std::vector<double> lognf(n_pairs), logng(n_pairs);
@@ -1056,13 +1114,15 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     for (int p = 0; p < n_pairs; ++p) {
         const wotk_batch_pair &q = pairs[p];
         const int ldi = (int)round_up(q.n_i, 4), ldj = (int)round_up(q.n_j, 4);
-        pf[p] = PairDev{q.c_offset, q.ldc, (int)q.n_i, (int)q.n_j, of * E, og * E, ldi, ldj, olw, p};
+        pf[p] = PairDev{q.c_offset, q.ldc, (int)q.n_i, (int)q.n_j, of * E, og * E, ldi, ldj, olw * E, p};
         pg[p] = PairDev{q.ct_offset, q.ldct, (int)q.n_j, (int)q.n_i, og * E, of * E, ldj, ldi, -1, p};
         goff[p] = q.g_offset;
-        lwoff[p] = olw;
+        lwoff[p] = olw * E;
         nsrc[p] = (int)q.n_i;
         lognf[p] = std::log((double)q.n_j);
         logng[p] = std::log((double)q.n_i);
+        run->n_i.push_back(q.n_i);
+        run->n_j.push_back(q.n_j);
         of += round_up(q.n_i, 4);
         og += round_up(q.n_j, 4);
         olw += q.n_i;
@@ -1077,14 +1137,21 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
         return t;
     };
     std::vector<Tile> tf = tiles_for(pf), tg = tiles_for(pg);
-    std::vector<IterParam> table((size_t)(n_iter + 1) * E);
-    for (int e = 0; e < E; ++e) {
-        auto reg = [&](int64_t n) { return (epsilon0 - epsilon[e]) * std::exp(-(double)n) + epsilon[e]; };
-        for (int64_t n = 0; n <= n_iter; ++n) {
-            const double eps = n < scaling_iter ? reg(n / inner_iter_max) : reg(scaling_iter / inner_iter_max);
-            table[(size_t)n * E + e] = IterParam{kLog2E / eps, eps, lambda1[e] / (lambda1[e] + eps), lambda2 / (lambda2 + eps)};
+    // eps(n) = (eps0 - eps) e^-n + eps, changed every inner_iter_max
+    // iterations, then extra_iter iterations at the last value
+    // (optimal_transport.py:184-185, 215-224, 231-233)
+    std::vector<IterParam> table((size_t)(n_iter + 1) * n_pairs * E);
+    run->eps_final.resize((size_t)n_pairs * E);
+    for (int p = 0; p < n_pairs; ++p)
+        for (int e = 0; e < E; ++e) {
+            const double ep = epsilon[p * E + e], l1 = lambda1[p * E + e], l2 = lambda2[p * E + e];
+            auto reg = [&](int64_t n) { return (epsilon0 - ep) * std::exp(-(double)n) + ep; };
+            for (int64_t n = 0; n <= n_iter; ++n) {
+                const double eps = n < scaling_iter ? reg(n / inner_iter_max) : reg(scaling_iter / inner_iter_max);
+                table[((size_t)n * n_pairs + p) * E + e] = IterParam{kLog2E / eps, eps, l1 / (l1 + eps), l2 / (l2 + eps)};
+            }
+            run->eps_final[(size_t)p * E + e] = table[((size_t)n_iter * n_pairs + p) * E + e].eps;
         }
-    }
     WOTK_HIP(hipMemcpyAsync(w.pairs_f, pf.data(), sizeof(PairDev) * n_pairs, hipMemcpyHostToDevice, s));
     WOTK_HIP(hipMemcpyAsync(w.pairs_g, pg.data(), sizeof(PairDev) * n_pairs, hipMemcpyHostToDevice, s));
     WOTK_HIP(hipMemcpyAsync(w.tiles_f, tf.data(), sizeof(Tile) * tf.size(), hipMemcpyHostToDevice, s));
@@ -1105,7 +1172,8 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     batch_fac_init_kernel<<<n_pairs, 256, 0, s>>>(w.ff, w.pairs_f, n_pairs, E);
     batch_fac_init_kernel<<<n_pairs, 256, 0, s>>>(w.fg, w.pairs_g, n_pairs, E);
     WOTK_LAUNCH_CHECK();
-    batch_growth_kernel<<<n_pairs, 256, 0, s>>>(G, w.g_off, w.n_src, w.logw, w.lw_off, w.logq);
+    batch_growth_kernel<<<dim3(n_pairs, E), 256, 0, s>>>(G, w.g_off, w.n_src, w.logw, w.lw_off, w.logq, E,
+                                                         growth_per_combo ? 1 : 0);
     batch_reset_kernel<<<1, 1, 0, s>>>(w.iter, w.queues, w.exact_rows, 0);
     batch_zero_f32_kernel<<<1, 256, 0, s>>>(w.fmax_f, n_pairs * E);
     batch_zero_f32_kernel<<<1, 256, 0, s>>>(w.fmax_g, n_pairs * E);
@@ -1113,89 +1181,139 @@ extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pai
     WOTK_HIP(hipStreamSynchronize(s));  // host tables above are released at return
 
     const int64_t n_tf = (int64_t)tf.size(), n_tg = (int64_t)tg.size();  // <= ntf, ntg (one-row tiles)
-    BatchArgs af{cost, w.pairs_f, w.tiles_f, (int)n_tf, w.table, w.iter, E, 0, w.f, w.g, w.logw, w.logq, w.lognf,
-                 w.queues, w.ff, w.fg, 0, w.exact_rows, w.bad_f, w.queues + 2, group, w.fmax_f, w.fmax_g,
-                 w.fmax_g, n_pairs * E};
-    BatchArgs ag{cost, w.pairs_g, w.tiles_g, (int)n_tg, w.table, w.iter, E, 1, w.g, w.f, w.logw, w.logq, w.logng,
-                 w.queues + 1, w.fg, w.ff, 1, w.exact_rows, w.bad_g, w.queues + 3, group, w.fmax_g, w.fmax_f,
-                 w.fmax_f, n_pairs * E};
+    run->af = BatchArgs{cost, w.pairs_f, w.tiles_f, (int)n_tf, w.table, w.iter, E, n_pairs, 0, w.f, w.g, w.logw,
+                        w.logq, w.lognf, w.queues, w.ff, w.fg, 0, w.exact_rows, w.bad_f, w.queues + 2, group,
+                        w.fmax_f, w.fmax_g, w.fmax_g, n_pairs * E};
+    run->ag = BatchArgs{cost, w.pairs_g, w.tiles_g, (int)n_tg, w.table, w.iter, E, n_pairs, 1, w.g, w.f, w.logw,
+                        w.logq, w.logng, w.queues + 1, w.fg, w.ff, 1, w.exact_rows, w.bad_g, w.queues + 3, group,
+                        w.fmax_g, w.fmax_f, w.fmax_f, n_pairs * E};
     const int grid = persistent_grid(*variant);
-    const int grid_f = (int)std::min<int64_t>(grid, n_tf), grid_g = (int)std::min<int64_t>(grid, n_tg);
-    const SweepFn sweep = variant->sweep;
+    run->grid_f = (int)std::min<int64_t>(grid, n_tf);
+    run->grid_g = (int)std::min<int64_t>(grid, n_tg);
+    run->sweep = variant->sweep;
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         ncu = 256;
-    const int grid_r = ncu;  // repair blocks: at most this many queued tiles in flight
-
-    hipEvent_t ev[2];
-    WOTK_HIP(hipEventCreate(&ev[0]));
-    WOTK_HIP(hipEventCreate(&ev[1]));
-    int64_t done = 0;
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    hipStream_t cs = nullptr;
+    run->grid_r = ncu;  // repair blocks: at most this many queued tiles in flight
+    run->E = E;
+    run->n_pairs = n_pairs;
+    run->group = group;
+    run->n_iter = n_iter;
+    run->graph_iters = graph_iters;
     if (graph_iters > 0 && n_iter >= graph_iters) {
         // capture graph_iters iterations once on a private stream, replay them
-        WOTK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-        WOTK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        WOTK_HIP(hipStreamCreateWithFlags(&run->cs, hipStreamNonBlocking));
+        WOTK_HIP(hipStreamBeginCapture(run->cs, hipStreamCaptureModeThreadLocal));
+        run->capturing = true;
         for (int t = 0; t < graph_iters; ++t) {
-            int rc = sweep(af, ag, grid_f, grid_g, grid_r, w.iter, w.queues, cs);
-            if (rc) {
-                hipStreamEndCapture(cs, &graph);
-                return rc;
-            }
+            const int rc = run->sweep(run->af, run->ag, run->grid_f, run->grid_g, run->grid_r, w.iter, w.queues, run->cs);
+            if (rc) return rc;  // the destructor ends the capture and drops the partial graph
         }
-        WOTK_HIP(hipStreamEndCapture(cs, &graph));
-        WOTK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        run->capturing = false;
+        WOTK_HIP(hipStreamEndCapture(run->cs, &run->graph));
+        WOTK_HIP(hipGraphInstantiate(&run->exec, run->graph, nullptr, nullptr, 0));
     }
-    WOTK_HIP(hipEventRecord(ev[0], s));
-    if (exec) {
-        while (n_iter - done >= graph_iters) {
-            WOTK_HIP(hipGraphLaunch(exec, s));
-            done += graph_iters;
+    *run_out = run.release();
+    return WOTK_OK;
+}
+
+extern "C" int wotk_batch_advance(wotk_batch_run *run, int64_t n_iterations, int64_t *remaining, void *stream) {
+    WOTK_REQUIRE(run != nullptr, "no batch run");
+    WOTK_REQUIRE(n_iterations >= 0, "negative iteration count");
+    hipStream_t s = (hipStream_t)stream;
+    int64_t todo = std::min<int64_t>(n_iterations, run->n_iter - run->done);
+    if (todo > 0) {
+        hipEvent_t a = nullptr, b = nullptr;
+        WOTK_HIP(hipEventCreate(&a));
+        run->timing.emplace_back(a, nullptr);
+        WOTK_HIP(hipEventCreate(&b));
+        run->timing.back().second = b;
+        WOTK_HIP(hipEventRecord(a, s));
+        while (run->exec && todo >= run->graph_iters) {
+            WOTK_HIP(hipGraphLaunch(run->exec, s));
+            run->done += run->graph_iters;
+            todo -= run->graph_iters;
         }
+        for (; todo > 0; --todo, ++run->done) {
+            const int rc = run->sweep(run->af, run->ag, run->grid_f, run->grid_g, run->grid_r, run->w.iter,
+                                      run->w.queues, s);
+            if (rc) return rc;
+        }
+        WOTK_HIP(hipEventRecord(b, s));
     }
-    for (; done < n_iter; ++done) {
-        int rc = sweep(af, ag, grid_f, grid_g, grid_r, w.iter, w.queues, s);
-        if (rc) return rc;
-    }
-    WOTK_HIP(hipEventRecord(ev[1], s));
+    if (remaining) *remaining = run->n_iter - run->done;
+    return WOTK_OK;
+}
+
+extern "C" int wotk_batch_potentials(wotk_batch_run *run, double *f_out, double *g_out, double *eps_final,
+                                     wotk_stats *stats, void *stream) {
+    WOTK_REQUIRE(run != nullptr, "no batch run");
+    hipStream_t s = (hipStream_t)stream;
+    const int E = run->E;
     // outputs: [pair][combo][n] contiguous blocks
-    {
-        long long oa = 0, ob = 0;
-        for (int p = 0; p < n_pairs; ++p) {
-            for (int e = 0; e < E; ++e) {
-                if (f_out)
-                    WOTK_HIP(hipMemcpyAsync(f_out + oa + (long long)e * pairs[p].n_i,
-                                            w.f + pf[p].pot_off + (long long)e * pf[p].pot_ld,
-                                            sizeof(double) * pairs[p].n_i, hipMemcpyDeviceToDevice, s));
-                if (g_out)
-                    WOTK_HIP(hipMemcpyAsync(g_out + ob + (long long)e * pairs[p].n_j,
-                                            w.g + pg[p].pot_off + (long long)e * pg[p].pot_ld,
-                                            sizeof(double) * pairs[p].n_j, hipMemcpyDeviceToDevice, s));
-            }
-            oa += (long long)E * pairs[p].n_i;
-            ob += (long long)E * pairs[p].n_j;
+    long long oa = 0, ob = 0;
+    for (int p = 0; p < run->n_pairs; ++p) {
+        const int64_t ni = run->n_i[p], nj = run->n_j[p];
+        for (int e = 0; e < E; ++e) {
+            if (f_out)
+                WOTK_HIP(hipMemcpyAsync(f_out + oa + (long long)e * ni,
+                                        run->w.f + run->pf[p].pot_off + (long long)e * run->pf[p].pot_ld,
+                                        sizeof(double) * ni, hipMemcpyDeviceToDevice, s));
+            if (g_out)
+                WOTK_HIP(hipMemcpyAsync(g_out + ob + (long long)e * nj,
+                                        run->w.g + run->pg[p].pot_off + (long long)e * run->pg[p].pot_ld,
+                                        sizeof(double) * nj, hipMemcpyDeviceToDevice, s));
         }
+        oa += (long long)E * ni;
+        ob += (long long)E * nj;
     }
-    unsigned long long exact_rows = 0;
-    WOTK_HIP(hipMemcpyAsync(&exact_rows, w.exact_rows, sizeof(exact_rows), hipMemcpyDeviceToHost, s));
-    WOTK_HIP(hipStreamSynchronize(s));
-    float ms = 0.f;
-    WOTK_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
-    if (cs) hipStreamDestroy(cs);
+    if (eps_final) std::copy(run->eps_final.begin(), run->eps_final.end(), eps_final);
     if (stats) {
+        unsigned long long exact_rows = 0;
+        WOTK_HIP(hipMemcpyAsync(&exact_rows, run->w.exact_rows, sizeof(exact_rows), hipMemcpyDeviceToHost, s));
+        WOTK_HIP(hipStreamSynchronize(s));
+        double ms = 0.0;
+        for (auto &ev : run->timing) {
+            float m = 0.f;
+            WOTK_HIP(hipEventElapsedTime(&m, ev.first, ev.second));
+            ms += m;
+        }
         *stats = wotk_stats{};
-        stats->iterations = n_iter;
-        stats->sweeps = 2 * n_iter;
+        stats->iterations = run->done;
+        stats->sweeps = 2 * run->done;
         stats->sweep_ms = ms;
-        stats->epsilon_final = table[(size_t)n_iter * E].eps;
+        stats->epsilon_final = run->eps_final[0];
         stats->stabilisations = (int64_t)exact_rows;
-        stats->gap_checks = group;
+        stats->gap_checks = run->group;
     }
     return WOTK_OK;
+}
+
+extern "C" void wotk_batch_end(wotk_batch_run *run) { delete run; }
+
+extern "C" int wotk_batch_stablev2(const float *cost, const wotk_batch_pair *pairs, int32_t n_pairs, const double *G,
+                                   int32_t n_combos, const double *epsilon, const double *lambda1, double lambda2,
+                                   int64_t scaling_iter, double tau, double epsilon0, int64_t extra_iter,
+                                   int64_t inner_iter_max, double *f_out, double *g_out, int32_t graph_iters,
+                                   int32_t flags, void *workspace, size_t workspace_bytes, wotk_stats *stats,
+                                   void *stream) {
+    (void)tau;  // absorptions are exact in the log domain (see sinkhorn.hip)
+    WOTK_REQUIRE(n_pairs > 0 && n_combos >= 1 && n_combos <= kMaxCombos && epsilon && lambda1,
+                 "empty batch or unsupported number of combinations");
+    // the grid is the same for every pair
+    std::vector<double> ep((size_t)n_pairs * n_combos), l1(ep.size()), l2(ep.size(), lambda2);
+    for (int p = 0; p < n_pairs; ++p)
+        for (int e = 0; e < n_combos; ++e) {
+            ep[(size_t)p * n_combos + e] = epsilon[e];
+            l1[(size_t)p * n_combos + e] = lambda1[e];
+        }
+    wotk_batch_run *raw = nullptr;
+    int rc = wotk_batch_begin(cost, pairs, n_pairs, G, 0, n_combos, ep.data(), l1.data(), l2.data(), scaling_iter,
+                              epsilon0, extra_iter, inner_iter_max, graph_iters, flags, workspace, workspace_bytes,
+                              &raw, stream);
+    if (rc) return rc;
+    std::unique_ptr<wotk_batch_run> run(raw);
+    rc = wotk_batch_advance(run.get(), scaling_iter + extra_iter, nullptr, stream);
+    if (rc) return rc;
+    return wotk_batch_potentials(run.get(), f_out, g_out, nullptr, stats, stream);
 }

@@ -33,7 +33,7 @@ class BatchResult:
         self.g = g                  # list per pair: tensor (E, J_p)
         self.stats = stats
         self.shapes = shapes
-        self.eps_final = eps_final  # per combination
+        self.eps_final = np.asarray(eps_final, dtype=np.float64)  # [pair][combination]
 
     def plan(self, p, e, device_out=False, out=None):
         """Transport map of pair p, combination e: exp((f+g-C)/eps) / J (FP64).
@@ -50,7 +50,7 @@ class BatchResult:
             device_out = True
         C = self.cost_buf[bp.c_offset:]
         _native.check(_native.lib().wotk_plan(_dev._ptr(C), bp.ldc, n_i, n_j, _dev._ptr(self.f[p][e]),
-                                              _dev._ptr(self.g[p][e]), float(self.eps_final[e]), float(n_j),
+                                              _dev._ptr(self.g[p][e]), float(self.eps_final[p, e]), float(n_j),
                                               _dev._ptr(out), out.stride(0), None, _dev._stream(torch)))
         return out if device_out else out.cpu().numpy()
 
@@ -62,7 +62,7 @@ class BatchResult:
         out = torch.empty(n_i, dtype=torch.float64, device=self.cost_buf.device)
         C = self.cost_buf[bp.c_offset:]
         _native.check(_native.lib().wotk_plan(_dev._ptr(C), bp.ldc, n_i, n_j, _dev._ptr(self.f[p][e]),
-                                              _dev._ptr(self.g[p][e]), float(self.eps_final[e]), float(n_j),
+                                              _dev._ptr(self.g[p][e]), float(self.eps_final[p, e]), float(n_j),
                                               None, n_j, _dev._ptr(out), _dev._stream(torch)))
         return out
 
@@ -151,47 +151,137 @@ def solve_time_course(coords, growth, epsilons=(0.05,), lambda1s=(1.0,), lambda2
                           graph_iters=graph_iters, group_size=group_size, share_exp=share_exp)
 
 
+class TimeCourseRun:
+    """One batched run advanced step by step (libwotk ``wotk_batch_begin`` /
+    ``wotk_batch_advance`` / ``wotk_batch_potentials`` / ``wotk_batch_end``).
+
+    ``epsilon``, ``lambda1``, ``lambda2``: arrays [pair][combination] (per-pair
+    parameters, wot/ot/initializer.py:135-183).  ``G``: a device float64
+    tensor, one growth vector per pair at ``bp.g_offset`` (``growth_per_combo``
+    False) or one per (pair, combination) ([combination][I_p] from
+    ``bp.g_offset``).  ``advance(n)`` runs the next n iterations of the
+    schedule; ``result()`` returns the potentials as a :class:`BatchResult`."""
+
+    def __init__(self, tc, epsilon, lambda1, lambda2, scaling_iter=3000, epsilon0=1.0, extra_iter=1000,
+                 inner_iter_max=50, graph_iters=50, share_exp=True, G=None, growth_per_combo=False,
+                 combos=None, bps=None):
+        torch = _dev.torch_mod()
+        from .. import tune
+        tune.apply_persisted()
+        self.tc = tc
+        self.device = tc.buf.device
+        eps = np.ascontiguousarray(epsilon, dtype=np.float64)
+        l1 = np.ascontiguousarray(lambda1, dtype=np.float64)
+        l2 = np.ascontiguousarray(lambda2, dtype=np.float64)
+        n_pairs = tc.n_pairs
+        if eps.ndim != 2 or eps.shape[0] != n_pairs or l1.shape != eps.shape or l2.shape != eps.shape:
+            raise ValueError('epsilon, lambda1, lambda2 must be [pair][combination] arrays of the same shape')
+        self.E = eps.shape[1]
+        self.n_iter = int(scaling_iter) + int(extra_iter)
+        self.combos = combos if combos is not None else [tuple(r) for r in np.stack([eps[0], l1[0]], 1)]
+        L = _native.lib()
+        self._bps = bps if bps is not None else tc.bps
+        arr = (_native.BatchPair * len(self._bps))(*self._bps)
+        self._arr = arr
+        self.ws = _dev.workspace(torch, L.wotk_batch_workspace_size(arr, n_pairs, self.E, self.n_iter), self.device)
+        self.G = tc.G if G is None else G
+        h = ctypes.c_void_p()
+        P = ctypes.POINTER(ctypes.c_double)
+        _native.check(L.wotk_batch_begin(_dev._ptr(tc.buf), arr, n_pairs, _dev._ptr(self.G), int(bool(growth_per_combo)),
+                                         self.E, eps.ctypes.data_as(P), l1.ctypes.data_as(P), l2.ctypes.data_as(P),
+                                         int(scaling_iter), float(epsilon0), int(extra_iter), int(inner_iter_max),
+                                         int(graph_iters), _native.BATCH_SHARE_EXP if share_exp else 0,
+                                         _dev._ptr(self.ws), self.ws.numel(), ctypes.byref(h), _dev._stream(torch)))
+        self._h = h
+        self.remaining = self.n_iter
+
+    def advance(self, n_iterations):
+        """Queue the next ``n_iterations`` (at most what is left); returns the iterations left."""
+        if self._h is None:
+            raise RuntimeError('batch run already closed')
+        torch = _dev.torch_mod()
+        left = ctypes.c_int64()
+        _native.check(_native.lib().wotk_batch_advance(self._h, int(n_iterations), ctypes.byref(left),
+                                                       _dev._stream(torch)))
+        self.remaining = int(left.value)
+        return self.remaining
+
+    def result(self):
+        """Potentials (and stats) at the current iteration as a BatchResult."""
+        torch = _dev.torch_mod()
+        tc, E = self.tc, self.E
+        shapes = tc.shapes
+        f_out = torch.empty(sum(E * s[0] for s in shapes), dtype=torch.float64, device=self.device)
+        g_out = torch.empty(sum(E * s[1] for s in shapes), dtype=torch.float64, device=self.device)
+        epsf = (ctypes.c_double * (tc.n_pairs * E))()
+        st = _native.Stats()
+        _native.check(_native.lib().wotk_batch_potentials(self._h, _dev._ptr(f_out), _dev._ptr(g_out), epsf,
+                                                          ctypes.byref(st), _dev._stream(torch)))
+        fs, gs = [], []
+        a = b = 0
+        for (n_i, n_j) in shapes:
+            fs.append(f_out[a:a + E * n_i].view(E, n_i))
+            gs.append(g_out[b:b + E * n_j].view(E, n_j))
+            a += E * n_i
+            b += E * n_j
+        d = st.as_dict()
+        d['exact_rows'] = d['stabilisations']  # rows recomputed with the online maximum
+        d['shared_group'] = d['gap_checks']     # combinations per shared exponential (0: none)
+        eps_pe = np.array(epsf[:], dtype=np.float64).reshape(tc.n_pairs, E)
+        return BatchResult(tc.buf, self._bps, self.combos, fs, gs, d, shapes, eps_pe)
+
+    def close(self):
+        if self._h is not None:
+            _native.lib().wotk_batch_end(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def grid_arrays(n_pairs, epsilons, lambda1s, lambda2):
+    """[pair][combination] parameter arrays of the Cartesian grid epsilons x
+    lambda1s (epsilon-major), the same for every pair."""
+    combos = list(itertools.product(epsilons, lambda1s))
+    eps = np.tile(np.array([c[0] for c in combos], dtype=np.float64), (n_pairs, 1))
+    l1 = np.tile(np.array([c[1] for c in combos], dtype=np.float64), (n_pairs, 1))
+    l2 = np.full_like(eps, float(lambda2))
+    return combos, eps, l1, l2
+
+
 def solve_prepared(tc, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_iter=3000, tau=1e4, epsilon0=1.0,
                    extra_iter=1000, inner_iter_max=50, graph_iters=50, group_size=None, share_exp=True):
     """``solve_time_course`` on costs already built by ``prepare_time_course``."""
     torch = _dev.torch_mod()
-    from .. import tune
-    tune.apply_persisted()
-    device = tc.buf.device
-    buf, bps, shapes, G, arr = tc.buf, tc.bps, tc.shapes, tc.G, tc.arr
-    combos = list(itertools.product(epsilons, lambda1s))
+    combos, eps, l1, l2 = grid_arrays(tc.n_pairs, epsilons, lambda1s, lambda2)
     E = len(combos)
-    n_iter = scaling_iter + extra_iter
-    L = _native.lib()
     gsz = E if group_size is None else int(group_size)
     groups = [list(range(k, min(E, k + gsz))) for k in range(0, E, gsz)]
-    ws = _dev.workspace(torch, max(L.wotk_batch_workspace_size(arr, len(bps), len(g), n_iter) for g in groups),
-                        device)
-    fs = [torch.empty((E, s[0]), dtype=torch.float64, device=device) for s in shapes]
-    gs = [torch.empty((E, s[1]), dtype=torch.float64, device=device) for s in shapes]
+    device = tc.buf.device
+    fs = [torch.empty((E, s[0]), dtype=torch.float64, device=device) for s in tc.shapes]
+    gs = [torch.empty((E, s[1]), dtype=torch.float64, device=device) for s in tc.shapes]
+    eps_final = np.zeros((tc.n_pairs, E))
     st_total = None
     for grp in groups:
-        Eg = len(grp)
-        f_out = torch.empty(sum(Eg * s[0] for s in shapes), dtype=torch.float64, device=device)
-        g_out = torch.empty(sum(Eg * s[1] for s in shapes), dtype=torch.float64, device=device)
-        eps = (ctypes.c_double * Eg)(*[combos[e][0] for e in grp])
-        lam = (ctypes.c_double * Eg)(*[combos[e][1] for e in grp])
-        st = _native.Stats()
-        _native.check(L.wotk_batch_stablev2(_dev._ptr(buf), arr, len(bps), _dev._ptr(G), Eg, eps, lam,
-                                            float(lambda2), int(scaling_iter), float(tau), float(epsilon0),
-                                            int(extra_iter), int(inner_iter_max), _dev._ptr(f_out), _dev._ptr(g_out),
-                                            int(graph_iters), _native.BATCH_SHARE_EXP if share_exp else 0,
-                                            _dev._ptr(ws), ws.numel(), ctypes.byref(st),
-                                            _dev._stream(torch)))
-        a = b = 0
-        for k, (n_i, n_j) in enumerate(shapes):
-            fs[k][grp] = f_out[a:a + Eg * n_i].view(Eg, n_i)
-            gs[k][grp] = g_out[b:b + Eg * n_j].view(Eg, n_j)
-            a += Eg * n_i
-            b += Eg * n_j
-        d = st.as_dict()
-        d['exact_rows'] = d['stabilisations']  # rows recomputed with the online maximum
-        d['shared_group'] = d['gap_checks']     # combinations per shared exponential (0: none)
+        with TimeCourseRun(tc, eps[:, grp], l1[:, grp], l2[:, grp], scaling_iter=scaling_iter, epsilon0=epsilon0,
+                           extra_iter=extra_iter, inner_iter_max=inner_iter_max, graph_iters=graph_iters,
+                           share_exp=share_exp, combos=[combos[e] for e in grp]) as run:
+            run.advance(run.n_iter)
+            r = run.result()
+        for k in range(tc.n_pairs):
+            fs[k][grp] = r.f[k]
+            gs[k][grp] = r.g[k]
+        eps_final[:, grp] = r.eps_final
+        d = r.stats
         if st_total is None:
             st_total = d
         else:
@@ -199,9 +289,5 @@ def solve_prepared(tc, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_
             st_total['sweeps'] += d['sweeps']
             st_total['stabilisations'] += d['stabilisations']
             st_total['exact_rows'] += d['exact_rows']
-
-    def reg(e, n):
-        return (epsilon0 - e) * np.exp(-n) + e
-    n_end = scaling_iter // inner_iter_max
-    eps_final = [reg(c[0], n_end) for c in combos]
-    return BatchResult(buf, bps, combos, fs, gs, st_total, shapes, eps_final)
+    del tau  # absorptions are exact in the log domain
+    return BatchResult(tc.buf, tc.bps, combos, fs, gs, st_total, tc.shapes, eps_final)
