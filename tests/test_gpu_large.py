@@ -78,3 +78,55 @@ def test_fused_row_sums_and_growth_loop(mods):
     assert np.abs(got - want).max() / np.abs(want).max() < 1e-6
     for a, b in zip(learned[1:], lg[1:]):
         assert np.abs(a.cpu().numpy() - b).max() / np.abs(b).max() < 1e-6
+
+
+def _vs_oracle(torch, large, x, y, G, terms, sched):
+    """Plan of the never-stored solve (rows regenerated with the FP64 cost)
+    against the FP64 oracle's transport_stablev2 on the reference cost."""
+    from oracle import ot_oracle as O
+    res = large.transport_stablev2_coords(x, y, G, terms=terms, **sched)
+    R = res.plan_rows(x, y, torch.arange(x.shape[0], device='cuda')).cpu().numpy()
+    Ro = O.transport_stablev2(O.default_cost_matrix(x, y), G=G, **sched)
+    err = np.linalg.norm(R - Ro) / np.linalg.norm(Ro)
+    marg = max(np.abs(R.sum(a) - Ro.sum(a)).max() / np.abs(Ro.sum(a)).max() for a in (0, 1))
+    return err, marg
+
+
+@pytest.mark.parametrize('n_i,n_j,dim,seed', [(700, 900, 50, 41), (97, 131, 7, 5)])
+def test_fused_vs_oracle(mods, n_i, n_j, dim, seed):
+    """The never-stored solve (6-term split-BF16 cross term) against the FP64
+    oracle restating wot/ot/optimal_transport.py:167-236 on the same inputs."""
+    torch, ot, dev, large = mods
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(n_i, n_j, dim=dim, n_components=5, seed=seed)
+    err, marg = _vs_oracle(torch, large, dp.x, dp.y, dp.growth, 6, SCHED)
+    print('fused vs oracle %dx%d: rel frob %.3e marginals %.3e' % (n_i, n_j, err, marg))
+    assert err <= 1e-5 and marg <= 1e-6
+
+
+def test_fused_outlier_vs_oracle(mods):
+    """eps0 = eps = 0.01 with an outlier source cell (its row is recomputed by
+    the repair kernel), against the FP64 oracle."""
+    torch, ot, dev, large = mods
+    from wot_amd.synthetic import time_course
+    coords, growth, _ = time_course(dim=10, n_components=4, seed=3, counts=[120, 140])
+    x = coords[0].copy()
+    x[0] += 8.0
+    sched = dict(SCHED, epsilon=0.01, epsilon0=0.01)
+    err, marg = _vs_oracle(torch, large, x, coords[1], growth[0], 6, sched)
+    assert err <= 1e-5 and marg <= 1e-6, (err, marg)
+
+
+def test_fused_nan_growth_propagates(mods):
+    """A NaN growth entry gives NaN potentials for its row (the reference's
+    arithmetic propagates it), never stale values."""
+    torch, ot, dev, large = mods
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(64, 80, dim=8, n_components=3, seed=12)
+    G = dp.growth.copy()
+    G[5] = np.nan
+    res = large.transport_stablev2_coords(dp.x, dp.y, G, terms=6, want_row_sums=True,
+                                          **dict(SCHED, scaling_iter=60, extra_iter=10))
+    f = res.f.cpu().numpy()
+    rs = res.row_sums.cpu().numpy()
+    assert np.isnan(f[5]) and np.isnan(rs[5])

@@ -280,3 +280,35 @@ def test_cost_bf16x6_error_and_plan_parity(ot):
     marg = max(np.abs(R.sum(ax) - Rref.sum(ax)).max() / np.abs(Rref.sum(ax)).max() for ax in (0, 1))
     print('bf16x6 plan rel frob %.3e marginals %.3e' % (rel, marg))
     assert rel <= PLAN_TOL and marg <= MARG_TOL
+
+
+def test_learnx_golden(ot):
+    """transport_stablev1_learnx (growth loop around transport_stablev2
+    semantics) against the reference-written growth-loop golden vectors."""
+    z = _load('ctm_stablev2')
+    prm = {k: v for k, v in z['params'].items() if k not in ('tolerance', 'max_iter', 'batch_size')}
+    R, learned = ot.transport_stablev1_learnx(C=z['C'], G=z['G'], **prm)
+    assert_parity(R, z['R'])
+    assert len(learned) == prm['growth_iters'] == len(z['learned'])
+    for got, want in zip(learned, z['learned']):
+        assert np.abs(np.asarray(got) - want).max() / np.abs(want).max() <= MARG_TOL
+
+
+def test_learnx_coordinates_never_stored(ot):
+    """transport_stablev1_learnx with coordinates only (the cost regenerated in
+    every sweep, never stored) against the oracle's growth loop on the
+    reference cost."""
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(150, 180, dim=20, n_components=4, seed=17)
+    prm = dict(lambda1=1.0, lambda2=50.0, epsilon=0.05, scaling_iter=600, tau=1e4, epsilon0=1.0, extra_iter=200,
+               inner_iter_max=50, growth_iters=2)
+    res, learned = ot.transport_stablev1_learnx(None, dp.growth, coords=(dp.x, dp.y), **prm)
+    Ro, lo = O.compute_transport_matrix(O.transport_stablev2, C=O.default_cost_matrix(dp.x, dp.y), G=dp.growth,
+                                        **prm)
+    assert_parity(res.dense().cpu().numpy(), Ro)
+    assert len(learned) == len(lo) == 2
+    for got, want in zip(learned, lo):
+        got = got.cpu().numpy() if hasattr(got, 'cpu') else np.asarray(got)
+        assert np.abs(got - want).max() / np.abs(want).max() <= MARG_TOL
+    with pytest.raises(ValueError, match='needs a cost matrix'):
+        ot.transport_stablev1_learnx(None, dp.growth, **prm)

@@ -204,8 +204,13 @@ struct FusedArgs {
 // are queued for the repair kernel (a NaN sum leaves the row as it is).
 __device__ inline void fused_row_epilogue(const FusedArgs &fa, int row, double S) {
     const double f_old = fa.pot[row];
+    if (isnan(S)) {  // NaN inputs (e.g. a NaN growth entry) propagate as in the reference's arithmetic
+        if (fa.row_sums) fa.row_sums[row] = S;
+        else fa.pot[row] = S;
+        return;
+    }
     if (!(S > 0.0) || isinf(S)) {
-        if (!isnan(S)) fa.bad_list[atomicAdd(fa.n_bad, 1)] = row;
+        fa.bad_list[atomicAdd(fa.n_bad, 1)] = row;
         return;
     }
     if (fa.row_sums) {  // the shift is the final potential: S = sum_j R_ij

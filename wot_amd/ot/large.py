@@ -14,9 +14,19 @@ from . import device as _dev
 
 
 class CoordsResult:
-    def __init__(self, f, g, median, stats, eps_final, row_sums=None):
+    def __init__(self, f, g, median, stats, eps_final, row_sums=None, x=None, y=None):
         self.f, self.g, self.median, self.stats, self.epsilon = f, g, median, stats, eps_final
         self.row_sums = row_sums  # row sums of the returned plan R / J
+        self.x, self.y = x, y     # the coordinates the plan is defined on
+
+    @property
+    def shape(self):
+        return (self.f.shape[0], self.g.shape[0])
+
+    def dense(self):
+        """The whole plan R / J (FP64, device): I x J x 8 bytes."""
+        torch = _dev.torch_mod()
+        return self.plan_rows(self.x, self.y, torch.arange(self.f.shape[0], device=self.f.device))
 
     def plan_rows(self, x, y, rows):
         """Rows of the plan R / J (FP64, exact FP64 cost) for the given source indices."""
@@ -54,7 +64,7 @@ def transport_stablev2_coords(x, y, G, lambda1=1.0, lambda2=50.0, epsilon=0.05, 
                                                    p(ws), ws.numel(), ctypes.byref(st),
                                                    ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
     stats = st.as_dict()
-    return CoordsResult(f, g, med, stats, stats['epsilon_final'], rs)
+    return CoordsResult(f, g, med, stats, stats['epsilon_final'], rs, X, Y)
 
 
 def transport_map_coords(x, y, G, growth_iters=1, **params):

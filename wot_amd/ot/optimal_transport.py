@@ -109,17 +109,32 @@ def compute_transport_matrix(solver, **params):
 
 
 def transport_stablev1_learnx(C, G, lambda1, lambda2, epsilon, scaling_iter, tau, epsilon0, extra_iter,
-                              inner_iter_max, growth_iters=1, device_out=False, **ignored):
+                              inner_iter_max, growth_iters=1, device_out=False, coords=None, terms=6, **ignored):
     """Growth-learning fixed-iteration transport (the ``learnx`` variant).
 
     The snapshot of wot this package follows no longer ships a
     ``transport_stablev1_learnx``; its contract here is: the growth outer loop
     of compute_transport_matrix (optimal_transport.py:10-33) around
     transport_stablev2 semantics (:167-236), learning the source growth from
-    the plan's row sums ``growth_iters`` times.  ``C`` may be a matrix or a
-    :class:`DeviceCost`.  Returns ``(tmap, learned_growth)`` where
-    ``learned_growth`` has ``growth_iters`` entries (the G used by each
-    solve)."""
+    the plan's row sums ``growth_iters`` times.  Returns ``(tmap,
+    learned_growth)`` where ``learned_growth`` has ``growth_iters`` entries
+    (the G used by each solve).
+
+    ``C`` may be a matrix or a :class:`DeviceCost`.  With ``C=None`` and
+    ``coords=(x, y)`` (source and target PCA coordinates) the cost
+    sqeuclid(x, y) / median (ot_model.py:242-253) is never stored: every sweep
+    regenerates it on the matrix cores (split-BF16 cross term with ``terms`` =
+    3 or 6 products), so a pair needs O(I + J) memory; ``tmap`` is then a
+    ``large.CoordsResult`` holding the potentials (``tmap.dense()`` or
+    ``tmap.plan_rows(rows)`` materialise R / J)."""
+    if C is None:
+        if coords is None:
+            raise ValueError('transport_stablev1_learnx needs a cost matrix C or coords=(x, y)')
+        from . import large
+        x, y = coords
+        return large.transport_map_coords(x, y, G, growth_iters=growth_iters, terms=terms, lambda1=lambda1,
+                                          lambda2=lambda2, epsilon=epsilon, scaling_iter=scaling_iter, tau=tau,
+                                          epsilon0=epsilon0, extra_iter=extra_iter, inner_iter_max=inner_iter_max)
     params = dict(C=C, G=G, lambda1=lambda1, lambda2=lambda2, epsilon=epsilon, scaling_iter=scaling_iter,
                   tau=tau, epsilon0=epsilon0, extra_iter=extra_iter, inner_iter_max=inner_iter_max,
                   growth_iters=growth_iters, device_out=device_out)
