@@ -120,6 +120,23 @@ int wotk_transport_stablev2(const float *C, int64_t ldc, const float *CT, int64_
                             double *row_sums, double *f_out, double *g_out, void *workspace,
                             size_t workspace_bytes, wotk_stats *stats, void *stream);
 
+/* No counterpart in the reference: the same fixed-iteration solve for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Arguments and outputs are
+ * those of wotk_transport_stablev2 without the workspace: the solver state
+ * lives in one static device array per GPU of wotk_scratch_capacity() bytes
+ * (16 MiB: wotk_solver_workspace_size(I, J) must fit, i.e. I, J up to about
+ * 95,000; larger problems are rejected with WOTK_EINVAL).  The call allocates
+ * nothing (no hipMalloc / hipFree, no events: stats->sweep_ms is 0); calls that
+ * overlap on one GPU share the array and race. */
+size_t wotk_scratch_capacity(void);
+int wotk_transport_stablev2_noalloc(const float *C, int64_t ldc, const float *CT, int64_t ldct, int64_t I,
+                                    int64_t J, const double *G, double lambda1, double lambda2,
+                                    double epsilon, int64_t scaling_iter, double tau, double epsilon0,
+                                    int64_t extra_iter, int64_t inner_iter_max, double *plan, int64_t ldp,
+                                    double *row_sums, double *f_out, double *g_out, wotk_stats *stats,
+                                    void *stream);
+
 /* optimal_transport_duality_gap (wot/ot/optimal_transport.py:67-164).  The
  * last epsilon scaling's gap passes run on a second, non-blocking stream that
  * the call creates and destroys (ordered after the caller's stream by an

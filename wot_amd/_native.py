@@ -83,6 +83,10 @@ SIGNATURES = {
     'wotk_transport_stablev2': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64, _D,
                                                _D, _I64, _I64, _P, _I64, _P, _P, _P, _P, _SZ,
                                                ctypes.POINTER(Stats), _P]),
+    'wotk_scratch_capacity': (_SZ, []),
+    'wotk_transport_stablev2_noalloc': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64, _D,
+                                                       _D, _I64, _I64, _P, _I64, _P, _P, _P,
+                                                       ctypes.POINTER(Stats), _P]),
     'wotk_transport_duality_gap': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64,
                                                   _D, _D, _D, _I64, _P, _I64, _P, _P, _P, _P, _SZ,
                                                   ctypes.POINTER(Stats), _P]),

@@ -19,6 +19,8 @@
 #include <array>
 #include <utility>
 
+#include <mutex>
+
 #include "common.h"
 
 namespace wotk {
@@ -668,6 +670,60 @@ static int setup(const float *C, int64_t ldc, const float *CT, int64_t ldct, int
     return init_state(*pb, *w, s);
 }
 
+// the two events bracketing a sweep loop; released on every exit path
+struct EventPair {
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~EventPair() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+// the fixed-iteration schedule on a set-up problem (shared by the workspace
+// and the scratch entry points); `timed` brackets the sweeps with events
+static int stablev2_run(const Problem &pb, Work &w, double lambda1, double lambda2, double epsilon,
+                        int64_t scaling_iter, double epsilon0, int64_t extra_iter, int64_t inner_iter_max,
+                        double *plan, int64_t ldp, double *row_sums, double *f_out, double *g_out, bool timed,
+                        wotk_stats *stats, hipStream_t s) {
+    WOTK_REQUIRE(scaling_iter >= 0 && extra_iter >= 0 && scaling_iter + extra_iter > 0,
+                 "at least one scaling iteration is required");
+    WOTK_REQUIRE(inner_iter_max > 0, "inner_iter_max must be positive");
+    WOTK_REQUIRE(!plan || ldp >= pb.n_j, "ldp < J");
+    auto reg = [&](int64_t n) { return (epsilon0 - epsilon) * std::exp(-(double)n) + epsilon; };
+    // the sweep loop is bracketed by two events on the launch stream so the
+    // caller gets the in-situ device time of the 2*(iterations) LSE sweeps
+    EventPair evs;
+    if (timed) {
+        WOTK_HIP(hipEventCreate(&evs.ev[0]));
+        WOTK_HIP(hipEventCreate(&evs.ev[1]));
+        WOTK_HIP(hipEventRecord(evs.ev[0], s));
+    }
+    int rc = 0;
+    for (int64_t n = 0; n < scaling_iter; ++n) {
+        rc = iterate(pb, w, lambda1, lambda2, reg(n / inner_iter_max), s);
+        if (rc) return rc;
+    }
+    const double eps_end = reg(scaling_iter / inner_iter_max);
+    for (int64_t n = 0; n < extra_iter; ++n) {
+        rc = iterate(pb, w, lambda1, lambda2, eps_end, s);
+        if (rc) return rc;
+    }
+    if (timed) WOTK_HIP(hipEventRecord(evs.ev[1], s));
+    rc = finish_outputs(pb, w, eps_end, (double)pb.n_j, plan, ldp, row_sums, f_out, g_out, s);
+    if (rc) return rc;
+    WOTK_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    if (timed) WOTK_HIP(hipEventElapsedTime(&ms, evs.ev[0], evs.ev[1]));
+    if (stats) {
+        *stats = wotk_stats{};
+        stats->iterations = scaling_iter + extra_iter;
+        stats->epsilon_final = eps_end;
+        stats->sweeps = 2 * (scaling_iter + extra_iter);
+        stats->sweep_ms = timed ? ms : 0.0;
+    }
+    return WOTK_OK;
+}
+
 extern "C" int wotk_transport_stablev2(const float *C, int64_t ldc, const float *CT, int64_t ldct,
                                        int64_t I, int64_t J, const double *G, double lambda1,
                                        double lambda2, double epsilon, int64_t scaling_iter,
@@ -681,43 +737,67 @@ extern "C" int wotk_transport_stablev2(const float *C, int64_t ldc, const float 
     Work w;
     int rc = setup(C, ldc, CT, ldct, I, J, G, workspace, workspace_bytes, &pb, &w, s);
     if (rc) return rc;
-    WOTK_REQUIRE(scaling_iter >= 0 && extra_iter >= 0 && scaling_iter + extra_iter > 0,
-                 "at least one scaling iteration is required");
-    WOTK_REQUIRE(inner_iter_max > 0, "inner_iter_max must be positive");
-    WOTK_REQUIRE(!plan || ldp >= J, "ldp < J");
     (void)tau;  // absorption is exact in the log domain; tau only changes the FP64 reference's rounding
-    auto reg = [&](int64_t n) { return (epsilon0 - epsilon) * std::exp(-(double)n) + epsilon; };
-    // the sweep loop is bracketed by two events on the launch stream so the
-    // caller gets the in-situ device time of the 2*(iterations) LSE sweeps
-    hipEvent_t ev[2];
-    WOTK_HIP(hipEventCreate(&ev[0]));
-    WOTK_HIP(hipEventCreate(&ev[1]));
-    WOTK_HIP(hipEventRecord(ev[0], s));
-    for (int64_t n = 0; n < scaling_iter; ++n) {
-        rc = iterate(pb, w, lambda1, lambda2, reg(n / inner_iter_max), s);
-        if (rc) return rc;
+    return stablev2_run(pb, w, lambda1, lambda2, epsilon, scaling_iter, epsilon0, extra_iter, inner_iter_max, plan,
+                        ldp, row_sums, f_out, g_out, true, stats, s);
+}
+
+// ---------------------------------------------------------------------------
+// The same solve for callers that cannot allocate: its state (potentials,
+// partial sums, counters) lives in one file-scope device array per GPU
+// instead of a caller workspace.  Every call on a GPU shares that GPU's array
+// (calls that overlap on one GPU race); the solver initialises every part of
+// the state it reads, so the previous call's contents do not matter.
+constexpr size_t kScratchBytes = (size_t)16 << 20;  // 16 MiB: I, J up to ~95,000
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static void *g_scratch_ptr[kMaxDevices];  // device address of `scratch` on each GPU (host-side)
+static std::mutex g_scratch_mutex;
+
+// the device address of this GPU's `scratch`, looked up once per device with
+// that device current (the array's host-side name is not a device address)
+static int scratch_ptr(void **out) {
+    int dev = 0;
+    WOTK_HIP(hipGetDevice(&dev));
+    WOTK_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        WOTK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        WOTK_REQUIRE(p != nullptr, "no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
     }
-    const double eps_end = reg(scaling_iter / inner_iter_max);
-    for (int64_t n = 0; n < extra_iter; ++n) {
-        rc = iterate(pb, w, lambda1, lambda2, eps_end, s);
-        if (rc) return rc;
-    }
-    WOTK_HIP(hipEventRecord(ev[1], s));
-    rc = finish_outputs(pb, w, eps_end, (double)J, plan, ldp, row_sums, f_out, g_out, s);
-    if (rc) return rc;
-    WOTK_HIP(hipStreamSynchronize(s));
-    float ms = 0.f;
-    WOTK_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-    WOTK_HIP(hipEventDestroy(ev[0]));
-    WOTK_HIP(hipEventDestroy(ev[1]));
-    if (stats) {
-        *stats = wotk_stats{};
-        stats->iterations = scaling_iter + extra_iter;
-        stats->epsilon_final = eps_end;
-        stats->sweeps = 2 * (scaling_iter + extra_iter);
-        stats->sweep_ms = ms;
-    }
+    *out = g_scratch_ptr[dev];
     return WOTK_OK;
+}
+
+extern "C" size_t wotk_scratch_capacity(void) { return kScratchBytes; }
+
+extern "C" int wotk_transport_stablev2_noalloc(const float *C, int64_t ldc, const float *CT, int64_t ldct,
+                                               int64_t I, int64_t J, const double *G, double lambda1,
+                                               double lambda2, double epsilon, int64_t scaling_iter,
+                                               double tau, double epsilon0, int64_t extra_iter,
+                                               int64_t inner_iter_max, double *plan, int64_t ldp,
+                                               double *row_sums, double *f_out, double *g_out,
+                                               wotk_stats *stats, void *stream) {
+    WOTK_REQUIRE(I > 0 && J > 0 && I < (1ll << 31) && J < (1ll << 31), "bad shape %lld x %lld", (long long)I,
+                 (long long)J);
+    const size_t need = wotk_solver_workspace_size(I, J);
+    WOTK_REQUIRE(need <= kScratchBytes,
+                 "%lld x %lld needs %zu bytes of solver state; the no-allocation entry point holds %zu "
+                 "(use wotk_transport_stablev2 with a workspace)",
+                 (long long)I, (long long)J, need, kScratchBytes);
+    hipStream_t s = (hipStream_t)stream;
+    void *ws = nullptr;
+    int rc = scratch_ptr(&ws);
+    if (rc) return rc;
+    Problem pb;
+    Work w;
+    rc = setup(C, ldc, CT, ldct, I, J, G, ws, kScratchBytes, &pb, &w, s);
+    if (rc) return rc;
+    (void)tau;
+    return stablev2_run(pb, w, lambda1, lambda2, epsilon, scaling_iter, epsilon0, extra_iter, inner_iter_max, plan,
+                        ldp, row_sums, f_out, g_out, false, stats, s);
 }
 
 // second stream and event of the duality-gap solver's last scaling; drained
