@@ -114,3 +114,59 @@ def test_otmodel_errors(amd):
         model.compute_transport_map(2.0, 3.0)
     assert model.day_pairs == {(1.0, 2.0): {'epsilon': 0.1}}
     assert ot.OTModel(adata, local_pca=0).compute_transport_map(1, 5) is None  # no cells at day 5
+
+
+def test_compute_all_transport_maps_batched_matches_sequential(amd, tmp_path):
+    """fixed_iters, per-pair epsilon / lambda1 / lambda2 from a day-pair
+    configuration, growth_iters=2: the batched run (all pairs in one device
+    run) writes the same maps, learned growth and _g.txt as the pair-by-pair
+    solves, and each map matches the FP64 oracle's growth loop."""
+    ot, _anndata = amd
+    from wot_amd.io import read_npz_tmap
+    from wot_amd.synthetic import time_course
+    coords, growth, counts = time_course(n_days=4, min_cells=60, max_cells=160, dim=8, n_components=3, seed=21)
+    X = np.vstack(coords)
+    days = np.repeat(np.arange(4, dtype=float), counts)
+    adata = _adata(_anndata, X, days, growth=np.concatenate(growth))
+    cfg = tmp_path / 'pairs.txt'
+    cfg.write_text('t0\tt1\tepsilon\tlambda1\tlambda2\n0\t1\t0.05\t1\t50\n1\t2\t0.1\t2\t40\n2\t3\t0.03\t0.5\t60\n')
+    prm = dict(local_pca=0, solver='fixed_iters', growth_iters=2, scaling_iter=400, extra_iter=100,
+               config=str(cfg))
+    out_b = os.path.join(str(tmp_path), 'b', 'tm')
+    out_s = os.path.join(str(tmp_path), 's', 'tm')
+    ot.OTModel(adata, **prm).compute_all_transport_maps(tmap_out=out_b, output_file_format='npz', batch=True)
+    ot.OTModel(adata, **prm).compute_all_transport_maps(tmap_out=out_s, output_file_format='npz', batch=False)
+    per_pair = {(0.0, 1.0): (0.05, 1.0, 50.0), (1.0, 2.0): (0.1, 2.0, 40.0), (2.0, 3.0): (0.03, 0.5, 60.0)}
+    for t in range(3):
+        name = '_{}_{}.npz'.format(float(t), float(t + 1))
+        Xb, obs_b, var_b = read_npz_tmap(out_b + name)
+        Xs, obs_s, var_s = read_npz_tmap(out_s + name)
+        assert list(obs_b.index) == list(obs_s.index) and list(var_b) == list(var_s)
+        assert np.linalg.norm(Xb - Xs) / np.linalg.norm(Xs) <= 1e-6
+        for g in ('g0', 'g1', 'g2'):
+            assert np.abs(obs_b[g].values - obs_s[g].values).max() / np.abs(obs_s[g].values).max() <= 1e-6
+        eps, l1, l2 = per_pair[(float(t), float(t + 1))]
+        Ro, _ = O.compute_transport_matrix(O.transport_stablev2, C=O.default_cost_matrix(coords[t], coords[t + 1]),
+                                           G=growth[t], growth_iters=2, epsilon=eps, lambda1=l1, lambda2=l2,
+                                           epsilon0=1.0, tau=1e4, scaling_iter=400, extra_iter=100,
+                                           inner_iter_max=50)
+        assert np.linalg.norm(Xb - Ro) / np.linalg.norm(Ro) <= 1e-5
+        for a in (0, 1):
+            assert np.abs(Xb.sum(a) - Ro.sum(a)).max() / np.abs(Ro.sum(a)).max() <= 1e-6
+    gb = pd.read_csv(out_b + '_g.txt', sep='\t', index_col=0)
+    gs = pd.read_csv(out_s + '_g.txt', sep='\t', index_col=0)
+    assert list(gb.index) == list(gs.index) and list(gb.columns) == ['g0', 'g1', 'g2']
+    np.testing.assert_allclose(gb.values, gs.values, rtol=1e-6)
+
+
+def test_compute_all_transport_maps_batch_refusals(amd, tmp_path):
+    """batch=True with the duality-gap solver or given cost matrices raises;
+    'auto' falls back to the pair-by-pair loop (same files)."""
+    ot, _anndata = amd
+    adata = _adata(_anndata, np.random.rand(9, 5), [1, 1, 1, 2, 2, 2, 3, 3, 3])
+    with pytest.raises(ValueError, match="cannot batch"):
+        ot.OTModel(adata, local_pca=0).compute_all_transport_maps(tmap_out=str(tmp_path / 'a' / 't'),
+                                                                  output_file_format='npz', batch=True)
+    ot.OTModel(adata, local_pca=0).compute_all_transport_maps(tmap_out=str(tmp_path / 'b' / 't'),
+                                                              output_file_format='npz')
+    assert sorted(os.listdir(str(tmp_path / 'b'))) == ['t_1_2.npz', 't_2_3.npz']

@@ -121,13 +121,21 @@ class OTModel:
         return itertools.product(covariate, covariate)
 
     def compute_all_transport_maps(self, tmap_out='tmaps', overwrite=True, output_file_format='h5ad',
-                                   with_covariates=False, cost_matrices=None):
+                                   with_covariates=False, cost_matrices=None, batch='auto'):
         """Computes and saves every consecutive (or configured) day pair
         (ot_model.py:124-201).  Existing outputs are skipped when
         ``overwrite`` is False; learned growth goes to ``<prefix>_g.txt``
         when ``growth_iters > 1``.  Each map is written by one background
         thread while the next pair is solved (at most one write in flight, so
-        one extra host copy of a map); write errors surface here."""
+        one extra host copy of a map); write errors surface here.
+
+        ``batch``: with the fixed_iters solver, no covariates and no given
+        cost matrices, every pair (each with its own epsilon / lambda1 /
+        lambda2 from the day-pair configuration) is solved in one batched run
+        on the device -- one persistent launch per half-iteration for all
+        pairs, growth iterations through the plans' row sums -- when the
+        pairs' FP32 costs fit in device memory ('auto', the default), always
+        (True) or never (False).  Results match the pair-by-pair solves."""
         from concurrent.futures import ThreadPoolExecutor
         from ..io import check_file_extension, write_dataset
         tmap_dir, tmap_prefix = os.path.split(tmap_out) if tmap_out is not None else (None, None)
@@ -144,15 +152,29 @@ class OTModel:
         if not day_pairs:
             logger.info('No day pairs')
             return
+        use_batch = batch is True or (batch == 'auto' and cost_matrices is None and not with_covariates)
         if cost_matrices is None:
             cost_matrices = [None for _ in day_pairs]
         full_learned_growth_df = None
         save_learned_growth = self.ot_config.get('growth_iters', 1) > 1
+        todo = []
+        for day_pair in day_pairs:
+            path = tmap_prefix + ("_{}_{}".format(*day_pair) if not with_covariates
+                                  else "_{}_{}_cv{}_cv{}".format(*day_pair))
+            todo.append(check_file_extension(os.path.join(tmap_dir, path), output_file_format))
+        batched = None
+        if use_batch:
+            batched = self._batched_maps(day_pairs, todo, overwrite, cost_matrices, with_covariates,
+                                         required=batch is True)
         with ThreadPoolExecutor(1) as writer:
             pending = None
-            for day_pair, cost_matrix in zip(day_pairs, cost_matrices):
-                tmap, output_file = self._next_map(day_pair, cost_matrix, with_covariates, tmap_dir, tmap_prefix,
-                                                   output_file_format, overwrite, check_file_extension)
+            for k, (day_pair, cost_matrix) in enumerate(zip(day_pairs, cost_matrices)):
+                if batched is not None:
+                    tmap, output_file = batched(k), todo[k]
+                else:
+                    tmap, output_file = self._next_map(day_pair, cost_matrix, with_covariates, tmap_dir,
+                                                       tmap_prefix, output_file_format, overwrite,
+                                                       check_file_extension)
                 if tmap is None:
                     continue
                 if pending is not None:
@@ -167,6 +189,102 @@ class OTModel:
         if full_learned_growth_df is not None:
             full_learned_growth_df.to_csv(os.path.join(tmap_dir, tmap_prefix + '_g.txt'), sep='\t',
                                           index_label='id')
+
+    def _batched_maps(self, day_pairs, outputs, overwrite, cost_matrices, with_covariates, required=False):
+        """Solve every pair of compute_all_transport_maps in one batched
+        device run (libwotk wotk_batch_begin / advance, per-pair parameters).
+        Returns a function k -> AnnData map of pair k (None: skipped, as the
+        sequential loop skips it), or None when the pairs cannot be batched
+        (then the caller solves them one by one); ``required`` turns that
+        into a ValueError."""
+        from . import batch as _batch
+
+        def refuse(why):
+            if required:
+                raise ValueError('compute_all_transport_maps cannot batch these pairs: ' + why)
+            logger.info('solving pairs one by one: %s', why)
+            return None
+
+        if self.solver is not _ot.transport_stablev2:
+            return refuse("batching runs the 'fixed_iters' solver")
+        if with_covariates or any(c is not None for c in cost_matrices):
+            return refuse('covariates or given cost matrices')
+        keys = ('growth_iters', 'scaling_iter', 'extra_iter', 'inner_iter_max', 'epsilon0', 'local_pca')
+        coords, G, prm, slots = [], [], [], {}
+        shared = None
+        for k, (t0, t1) in enumerate(day_pairs):
+            if os.path.exists(outputs[k]) and not overwrite:
+                logger.info('Found existing tmap at ' + outputs[k] + '. ')
+                continue
+            local = self.day_pairs[(t0, t1)] if self.day_pairs is not None else {}
+            if self.day_pairs is not None and (t0, t1) not in self.day_pairs:
+                raise ValueError("Transport map ({},{}) is not present in day_pairs".format(t0, t1))
+            cfg = {**self.ot_config, **local}
+            sh = tuple(cfg.get(x) for x in keys)
+            if shared is None:
+                shared = sh
+            elif sh != shared:
+                return refuse('pairs differ in ' + ', '.join(x for x, a, b in zip(keys, sh, shared) if a != b))
+            day = self.matrix.obs[self.day_field]
+            p0 = self.matrix[(day == float(t0)).values, :]
+            p1 = self.matrix[(day == float(t1)).values, :]
+            if p0.shape[0] == 0 or p1.shape[0] == 0:
+                logger.info('No cells at {}'.format(t0 if p0.shape[0] == 0 else t1))
+                continue
+            if cfg['local_pca'] is not None and cfg['local_pca'] > 0:
+                x, y, pca, _ = compute_pca(p0.X, p1.X, cfg['local_pca'])
+                ev = np.diag(pca.singular_values_)
+                x, y = np.asarray(x).dot(ev), np.asarray(y).dot(ev)
+            else:
+                x, y = _dense(p0.X), _dense(p1.X)
+            dt = t1 - t0
+            if self.cell_growth_rate_field in p0.obs.columns:
+                g = np.power(p0.obs[self.cell_growth_rate_field].values.astype(np.float64), dt)
+            else:
+                g = np.ones(p0.shape[0])
+            slots[k] = (len(prm), p0.obs.index, p1.obs.index, dt)
+            coords += [np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64)]
+            G.append(g)
+            prm.append((cfg['epsilon'], cfg['lambda1'], cfg['lambda2']))
+        if not prm:
+            return lambda k: None
+        torch = _dev.torch_mod()
+        n = len(prm)
+        need = sum(8.0 * c.shape[0] * coords[2 * i + 1].shape[0] for i, c in enumerate(coords[0::2]))
+        free, _ = torch.cuda.mem_get_info()
+        if need > 0.7 * free:
+            return refuse('the %d costs need %.1f GB of the %.1f GB free' % (n, need / 1e9, free / 1e9))
+        growth_iters, scaling_iter, extra_iter, inner_iter_max, epsilon0, _ = shared
+        tc = _batch.prepare_time_course(coords, [G[i // 2] if i % 2 == 0 else None for i in range(2 * n)],
+                                        pairs=[(2 * i, 2 * i + 1) for i in range(n)], cost_mode=self.cost_mode)
+        eps = np.array([[p[0]] for p in prm], dtype=np.float64)
+        l1 = np.array([[p[1]] for p in prm], dtype=np.float64)
+        l2 = np.array([[p[2]] for p in prm], dtype=np.float64)
+        learned = [[torch.from_numpy(g).cuda()] for g in G]
+        Gk = tc.G
+        res = None
+        for it in range(int(growth_iters)):
+            if it > 0:
+                Gk = torch.cat([res.row_sums(i, 0) for i in range(n)])
+                for i in range(n):
+                    learned[i].append(res.row_sums(i, 0))
+            with _batch.TimeCourseRun(tc, eps, l1, l2, scaling_iter=scaling_iter, epsilon0=epsilon0,
+                                      extra_iter=extra_iter, inner_iter_max=inner_iter_max, G=Gk) as run:
+                run.advance(run.n_iter)
+                res = run.result()
+
+        def get(k):
+            if k not in slots:
+                return None
+            i, obs_ids, var_ids, dt = slots[k]
+            R = res.plan(i, 0, device_out=True)
+            rows = learned[i] + [R.sum(1)]
+            obs = pd.DataFrame(index=obs_ids, data={'g' + str(j): np.power(r.cpu().numpy(), 1.0 / dt)
+                                                    for j, r in enumerate(rows)})
+            out = _anndata.AnnData(R.cpu().numpy(), obs, pd.DataFrame(index=var_ids))
+            del R
+            return out
+        return get
 
     def _next_map(self, day_pair, cost_matrix, with_covariates, tmap_dir, tmap_prefix, output_file_format,
                   overwrite, check_file_extension):
