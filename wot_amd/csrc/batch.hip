@@ -496,17 +496,24 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
 // 256 columns (double-buffered, loaded once per block instead of once per
 // wave), so their L2 traffic is divided by the rows per tile.  Arithmetic,
 // range checks and epilogue are those of batch_shared_kernel.
-// Columns staged per chunk (multiple of 256) and LDS buffers (2: two barriers
-// per chunk; 3: one), and the kernel is compiled for 3 waves per SIMD (the
-// register cap spills a few values the chunk loop does not touch).  Measured on
-// the 38-pair, 3 x 3 grid time course, ms per sweep with 4 waves x 2 rows /
-// 4 waves x 4 rows: 512/3 at 2 waves per SIMD 5.32 / 10.6; 512/2 at 3 waves
-// 5.26 / 5.88; 256/3 at 3 waves 5.18 / 5.11; 256/2 at 3 waves 5.92 / 5.11.
+// One 256-column chunk per step and three LDS buffers (one barrier per
+// chunk).  Each wave loads its rows' cost entries for the next chunk into
+// registers before computing the current one, so the HBM latency is hidden
+// behind the chunk's ~600 VALU instructions instead of stalling every wave of
+// the block after the barrier; the kernel is compiled for 2 waves per SIMD,
+// which gives it 256 VGPRs and no spills.  Measured on the 38-pair, 3 x 3
+// grid time course, us per sweep for 4 waves x 4 rows: no prefetch at 3 waves
+// per SIMD 5850 (spills 268 bytes); no prefetch at 2 waves 6800; prefetch at
+// 3 waves 6470 (spills 352 bytes); prefetch at 2 waves 4880-4940.  (Earlier,
+// without prefetch: 512-column chunks at 2 waves 5.32 / 10.6 ms for 4 x 2 /
+// 4 x 4 rows, 256/2 buffers at 3 waves 5.92 / 5.11 ms.)  Dropping the per-entry
+// clamp of the split exponent (clamping the potentials where they are staged
+// instead) measured slower again with prefetch: 5117 us.
 constexpr int kLdsChunk = 256;
 constexpr int kLdsBuffers = 3;
 
 template <int NG, int NL, int WROWS, int NWAVES, int CH = kLdsChunk, int NBUF = kLdsBuffers>
-__global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 8))) void batch_shared_lds_kernel(
+__global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 8))) void batch_shared_lds_kernel(
     BatchArgs ba) {
     constexpr int E = NG * NL, L0 = NL / 2, THREADS = NWAVES * 64, RE = WROWS * E;
     constexpr int NB = NL - 1;                        // factor columns per group
@@ -590,6 +597,10 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 
     }
         WOTK_LDS_FETCH(0)
         WOTK_LDS_COMMIT(0)
+        static_assert(CH == 256, "prefetch assumes one 256-column step per chunk");
+        float4 cnext[WROWS];
+#pragma unroll
+        for (int r = 0; r < WROWS; ++r) cnext[r] = lane * 4 < ncols4 ? bload4(rowp[r] + lane * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
         for (int k = 0; k < nchunks; ++k) {
             const int buf = k % NBUF;
@@ -600,16 +611,23 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 
             for (int sub = 0; sub < CH / 256; ++sub) {
             const int cl = sub * 256 + lane * 4;  // column within the chunk
             const int c = k * CH + cl;
+            float4 ccur[WROWS];
+#pragma unroll
+            for (int r = 0; r < WROWS; ++r) ccur[r] = cnext[r];
+            if (k + 1 < nchunks && (k + 1) * CH + cl < ncols4) {
+#pragma unroll
+                for (int r = 0; r < WROWS; ++r) cnext[r] = bload4(rowp[r] + (k + 1) * CH + cl);
+            }
             if (c < ncols4) {
                 double cv[WROWS][4];
 #pragma unroll
                 for (int r = 0; r < WROWS; ++r) {
-                    const float4 a = bload4(rowp[r] + c);
-                    cv[r][0] = a.x;
-                    cv[r][1] = a.y;
-                    cv[r][2] = a.z;
-                    cv[r][3] = a.w;
+                    cv[r][0] = ccur[r].x;
+                    cv[r][1] = ccur[r].y;
+                    cv[r][2] = ccur[r].z;
+                    cv[r][3] = ccur[r].w;
                 }
+
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
                     const double2 h01 = *reinterpret_cast<const double2 *>(&h_lds[buf][g][cl]);
