@@ -221,7 +221,7 @@ def valu_rates(torch):
     sink = torch.empty(L.wotk_probe_sink_bytes() // 4, dtype=torch.float32, device='cuda')
     rates = (ctypes.c_double * 7)()
     _native.check(L.wotk_probe_valu_rates(rates, 7, dev._ptr(sink), sink.numel() * 4, dev._stream(torch)))
-    names = ['exp_f32', 'fma_f64', 'fma_f32', 'cvt_f64_f32', 'rndne_f64_add', 'ldexp_f32', 'split_exp_entry']
+    names = ['exp_f32', 'fma_f64', 'fma_f32', 'add_f64_cvt_pair', 'rndne_f64_add', 'ldexp_f32', 'split_exp_entry']
     return {n: float(v) for n, v in zip(names, rates)}
 
 

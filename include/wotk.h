@@ -430,7 +430,7 @@ int wotk_get_tile(int32_t kernel, int32_t bucket, wotk_tile *out);
  * the issue rate of the instructions the compute-bound batched sweep is built
  * from, for its roofline.  rates_host (HOST, >= 7 entries) receives, in
  * operations per second over the whole device: [0] v_exp_f32, [1] FP64 FMA,
- * [2] FP32 FMA, [3] FP64<->FP32 converts, [4] FP64 round-to-nearest (+ add),
+ * [2] FP32 FMA, [3] FP64 add + FP64->FP32 + FP32->FP64 converts, [4] FP64 round-to-nearest (+ add),
  * [5] v_ldexp_f32, [6] the sweep's exact split exponential of an FP64
  * exponent (one whole entry: 2 FP64 FMAs, clamp, rint, subtract, two
  * converts, exp2, ldexp).  sink: device scratch of wotk_probe_sink_bytes()

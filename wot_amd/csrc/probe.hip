@@ -19,7 +19,7 @@ enum ProbeOp {
     PROBE_EXP_F32 = 0,    // v_exp_f32
     PROBE_FMA_F64 = 1,    // v_fma_f64
     PROBE_FMA_F32 = 2,    // v_fma_f32
-    PROBE_CVT_F64 = 3,    // v_cvt_f32_f64 + v_cvt_f64_f32 (a pair)
+    PROBE_CVT_F64 = 3,    // v_add_f64 + v_cvt_f32_f64 + v_cvt_f64_f32 (a triple)
     PROBE_RNDNE_F64 = 4,  // v_rndne_f64 + v_add_f64 (a pair)
     PROBE_LDEXP_F32 = 5,  // v_ldexp_f32
     PROBE_SPLIT_EXP = 6,  // the sweep's exact split exponential of an FP64 exponent (one entry)
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void probe_kernel(float *sink, float seed) {
             } else if constexpr (OP == PROBE_FMA_F32) {
                 xf[c] = fmaf(xf[c], 0.999f, 1.0e-3f);
             } else if constexpr (OP == PROBE_CVT_F64) {
-                xd[c] = (double)(float)xd[c];
+                xd[c] = (double)(float)(xd[c] + 1.0e-7);  // the add keeps the converts from folding
             } else if constexpr (OP == PROBE_RNDNE_F64) {
                 xd[c] = rint(xd[c]) + 0.25;
             } else if constexpr (OP == PROBE_LDEXP_F32) {
@@ -69,8 +69,8 @@ __global__ __launch_bounds__(256) void probe_kernel(float *sink, float seed) {
     if (acc == -1.0f) sink[tid] = acc;  // never true for these chains
 }
 
-// operations per step and chain of each probe (the pairs count as two)
-constexpr int kProbeOpsPerStep[PROBE_NUM] = {1, 1, 1, 2, 2, 1, 1};
+// operations per step and chain of each probe (pairs count as two, the convert triple as three)
+constexpr int kProbeOpsPerStep[PROBE_NUM] = {1, 1, 1, 3, 2, 1, 1};
 
 template <int OP>
 static int probe_time(float *sink, int grid, hipStream_t s, double *ops_per_s) {
