@@ -20,111 +20,120 @@ int launch_plan_sums(const float *A, int64_t lda, int nrows, int ncols, const do
                      double eps, double *row_sums, hipStream_t s);
 
 constexpr int kAdjThreads = 256;
-constexpr int kAdjRows = 4;
+constexpr int kAdjRows = 16;  // rows per block
+constexpr int kAdjCols = 32;  // columns per chunk
 
-// W = (R - K~)/(IJ) written in FP64; per row: sum R, sum W; per block: sum W*C
+// One pass over the cost rows: W_ij = (R_ij - K~_ij)/(IJ) for a block of 16
+// rows and a chunk of 32 columns is formed in FP64 (two exps per entry) into
+// LDS next to the chunk's 32 target coordinates, and the block accumulates
+// V = W Y (rows x d, FP64, registers) from LDS, so W never reaches HBM.  Per
+// row it also sums R and W, and per block W * C.  Thread t forms entries
+// (row t / 32 + 8 m, column t % 32) of a chunk and owns V[2 (t / 32) + a][t % 32
+// + 32 q] (a < 2, q < KPT, d <= 32 KPT).
+template <int KPT>
 __global__ __launch_bounds__(kAdjThreads) void adjoint_rows_kernel(
     const float *__restrict__ A, int64_t lda, int n_i, int n_j, const double *__restrict__ f,
-    const double *__restrict__ g, double inv_eps, double inv_ij, double *__restrict__ W,
-    double *__restrict__ r_out, double *__restrict__ sw_out, double *__restrict__ partials) {
+    const double *__restrict__ g, double inv_eps, double inv_ij, const double *__restrict__ Y, int d,
+    double *__restrict__ V, double *__restrict__ r_out, double *__restrict__ sw_out,
+    double *__restrict__ partials) {
+    constexpr int DP = 32 * KPT;
+    __shared__ double w_lds[kAdjRows][kAdjCols + 1];  // +1: rows of the FMA phase fall on distinct banks
+    __shared__ double y_lds[kAdjCols][DP];
+    __shared__ double red[2 * kAdjRows + 1][kAdjThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r0 = blockIdx.x * kAdjRows;
-    const int tid = threadIdx.x;
-    double acc_r[kAdjRows], acc_w[kAdjRows], fr[kAdjRows];
-    const float *rowp[kAdjRows];
-    double s_wc = 0.0;
+    const int jc = tid % kAdjCols, rq = tid / kAdjCols;  // chunk column; first row of this thread (step 8)
+    constexpr int RPT = kAdjRows / (kAdjThreads / kAdjCols);  // rows formed per thread: 2
+    double fr[RPT], acc_r[RPT], acc_w[RPT];
+    const float *rowp[RPT];
 #pragma unroll
-    for (int r = 0; r < kAdjRows; ++r) {
-        const int row = min(r0 + r, n_i - 1);
-        rowp[r] = A + (int64_t)row * lda;
-        fr[r] = f[row];
-        acc_r[r] = 0.0;
-        acc_w[r] = 0.0;
+    for (int m = 0; m < RPT; ++m) {
+        const int row = min(r0 + rq + 8 * m, n_i - 1);
+        rowp[m] = A + (int64_t)row * lda;
+        fr[m] = f[row];
+        acc_r[m] = 0.0;
+        acc_w[m] = 0.0;
     }
-    for (int j = tid; j < n_j; j += kAdjThreads) {
-        const double gj = g[j];
+    double s_wc = 0.0;
+    double v[2][KPT];
 #pragma unroll
-        for (int r = 0; r < kAdjRows; ++r) {
-            if (r0 + r >= n_i) continue;
-            const double c = (double)rowp[r][j];
-            const double R = exp((fr[r] + gj - c) * inv_eps);
-            const double w = (R - exp(-c * inv_eps)) * inv_ij;
-            W[(int64_t)(r0 + r) * n_j + j] = w;
-            acc_r[r] += R;
-            acc_w[r] += w;
-            s_wc = fma(w, c, s_wc);
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) v[a][q] = 0.0;
+    const int vr = 2 * rq;  // this thread's two V rows (block-local)
+    for (int j0 = 0; j0 < n_j; j0 += kAdjCols) {
+        const int j = j0 + jc;
+        const bool jin = j < n_j;
+        const double gj = jin ? g[j] : 0.0;
+#pragma unroll
+        for (int m = 0; m < RPT; ++m) {
+            double w = 0.0;
+            if (jin && r0 + rq + 8 * m < n_i) {
+                const double c = (double)rowp[m][j];
+                const double R = exp((fr[m] + gj - c) * inv_eps);
+                w = (R - exp(-c * inv_eps)) * inv_ij;
+                acc_r[m] += R;
+                acc_w[m] += w;
+                s_wc = fma(w, c, s_wc);
+            }
+            w_lds[rq + 8 * m][jc] = w;
+        }
+        for (int e = tid; e < kAdjCols * DP; e += kAdjThreads) {
+            const int jj = e / DP, k = e % DP;
+            y_lds[jj][k] = (j0 + jj < n_j && k < d) ? Y[(int64_t)(j0 + jj) * d + k] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int jj = 0; jj < kAdjCols; ++jj) {
+            const double w0 = w_lds[vr][jj], w1 = w_lds[vr + 1][jj];
+#pragma unroll
+            for (int q = 0; q < KPT; ++q) {
+                const double y = y_lds[jj][jc + 32 * q];
+                v[0][q] = fma(w0, y, v[0][q]);
+                v[1][q] = fma(w1, y, v[1][q]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int i = r0 + vr + a;
+        if (i >= n_i) continue;
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) {
+            const int k = jc + 32 * q;
+            if (k < d) V[(int64_t)i * d + k] = v[a][q];
         }
     }
-    __shared__ double red[2 * kAdjRows + 1][kAdjThreads / 64];
-    const int lane = tid & 63, wave = tid >> 6;
-    auto wsum = [](double v) {
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        return v;
+    auto wsum = [](double x) {
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        return x;
+    };
+    // a wave holds rows rq in {2 wave, 2 wave + 1} (+ 8 m): lanes 0-31 and 32-63
+    // form the two rows' columns; reduce each half separately
+    auto halfsum = [](double x) {
+        for (int off = 16; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        return x;
     };
 #pragma unroll
-    for (int r = 0; r < kAdjRows; ++r) {
-        const double a = wsum(acc_r[r]), b = wsum(acc_w[r]);
-        if (lane == 0) {
-            red[r][wave] = a;
-            red[kAdjRows + r][wave] = b;
+    for (int m = 0; m < RPT; ++m) {
+        const double a = halfsum(acc_r[m]), b = halfsum(acc_w[m]);
+        if ((lane & 31) == 0) {
+            red[rq + 8 * m][0] = a;
+            red[kAdjRows + rq + 8 * m][0] = b;
         }
     }
     const double t = wsum(s_wc);
     if (lane == 0) red[2 * kAdjRows][wave] = t;
     __syncthreads();
-    if (tid < 2 * kAdjRows + 1) {
-        double v = 0.0;
-        for (int w = 0; w < kAdjThreads / 64; ++w) v += red[tid][w];
-        if (tid < kAdjRows) {
-            if (r0 + tid < n_i) r_out[r0 + tid] = v;
-        } else if (tid < 2 * kAdjRows) {
-            if (r0 + tid - kAdjRows < n_i) sw_out[r0 + tid - kAdjRows] = v;
-        } else {
-            partials[blockIdx.x] = v;
-        }
-    }
-}
-
-// V = W Y : (I x J) (J x d) -> (I x d), FP64, 64 x 64 output tile per block,
-// J streamed through LDS in chunks of 32.
-__global__ __launch_bounds__(256) void gemm_wy_kernel(const double *__restrict__ W, int n_i, int n_j,
-                                                      const double *__restrict__ Y, int d,
-                                                      double *__restrict__ V) {
-    __shared__ double ws[32][64];
-    __shared__ double ys[32][64];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
-    double acc[4][4] = {};
-    for (int j0 = 0; j0 < n_j; j0 += 32) {
-        for (int e = tid; e < 64 * 32; e += 256) {
-            const int r = e / 32, jj = e % 32;  // W: row r, column j0+jj (coalesced along jj)
-            ws[jj][r] = (i0 + r < n_i && j0 + jj < n_j) ? W[(int64_t)(i0 + r) * n_j + j0 + jj] : 0.0;
-            const int jy = e / 64, kk = e % 64;  // Y: row j0+jy, column k0+kk
-            ys[jy][kk] = (j0 + jy < n_j && k0 + kk < d) ? Y[(int64_t)(j0 + jy) * d + k0 + kk] : 0.0;
-        }
-        __syncthreads();
-        for (int jj = 0; jj < 32; ++jj) {
-            double wv[4], yv[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                wv[q] = ws[jj][ty * 4 + q];
-                yv[q] = ys[jj][tx * 4 + q];
-            }
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = fma(wv[a], yv[b], acc[a][b]);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int i = i0 + ty * 4 + a;
-        if (i >= n_i) continue;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int k = k0 + tx * 4 + b;
-            if (k < d) V[(int64_t)i * d + k] = acc[a][b];
-        }
+    if (tid < kAdjRows) {
+        if (r0 + tid < n_i) r_out[r0 + tid] = red[tid][0];
+    } else if (tid < 2 * kAdjRows) {
+        if (r0 + tid - kAdjRows < n_i) sw_out[r0 + tid - kAdjRows] = red[tid][0];
+    } else if (tid == 2 * kAdjRows) {
+        double x = 0.0;
+        for (int w = 0; w < kAdjThreads / 64; ++w) x += red[2 * kAdjRows][w];
+        partials[blockIdx.x] = x;
     }
 }
 
@@ -209,12 +218,11 @@ __global__ void adjoint_median_kernel(const double *__restrict__ X, const double
 }
 
 struct AdjWork {
-    double *W, *r, *sw, *c, *V, *partials, *scal;
+    double *r, *sw, *c, *V, *partials, *scal;
     int nparts;
 };
 
 static size_t adj_carve(Arena &ar, int64_t I, int64_t J, int64_t d, AdjWork *w) {
-    w->W = ar.take<double>((size_t)I * J);
     w->r = ar.take<double>(I);
     w->sw = ar.take<double>(I);
     w->c = ar.take<double>(J);
@@ -245,24 +253,28 @@ extern "C" int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int
     WOTK_REQUIRE(I > 0 && J > 0 && d > 0 && I < (1ll << 31) && J < (1ll << 31), "bad shape");
     WOTK_REQUIRE(ldc >= J && ldct >= I, "leading dimension too small");
     WOTK_REQUIRE(X && Y && G && f && g && median, "missing input");
+    WOTK_REQUIRE(d <= 128, "the adjoint keeps a row's gradient in registers: d <= 128");
     Arena ar(workspace, workspace_bytes);
     AdjWork w;
     adj_carve(ar, I, J, d, &w);
     if (!workspace || !ar.ok) return fail(WOTK_EWORKSPACE, "adjoint workspace too small");
     const double inv_eps = 1.0 / epsilon;
     const double inv_ij = 1.0 / ((double)I * (double)J);
-    adjoint_rows_kernel<<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps, inv_ij, w.W, w.r,
-                                                        w.sw, w.partials);
+    switch ((d + 31) / 32) {
+        case 1: adjoint_rows_kernel<1><<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps, inv_ij, Y,
+                                                                        (int)d, w.V, w.r, w.sw, w.partials); break;
+        case 2: adjoint_rows_kernel<2><<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps, inv_ij, Y,
+                                                                        (int)d, w.V, w.r, w.sw, w.partials); break;
+        case 3: adjoint_rows_kernel<3><<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps, inv_ij, Y,
+                                                                        (int)d, w.V, w.r, w.sw, w.partials); break;
+        default: adjoint_rows_kernel<4><<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps, inv_ij, Y,
+                                                                         (int)d, w.V, w.r, w.sw, w.partials); break;
+    }
     WOTK_LAUNCH_CHECK();
     int rc = launch_plan_sums(CT, ldct, (int)J, (int)I, g, f, epsilon, w.c, s);
     if (rc) return rc;
     adjoint_scalars_kernel<<<1, 1024, 0, s>>>(w.partials, w.nparts, w.c, (int)I, (int)J, G, lambda2, w.scal);
     WOTK_LAUNCH_CHECK();
-    if (grad_x) {
-        dim3 grid((unsigned)((I + 63) / 64), (unsigned)((d + 63) / 64));
-        gemm_wy_kernel<<<grid, 256, 0, s>>>(w.W, (int)I, (int)J, Y, (int)d, w.V);
-        WOTK_LAUNCH_CHECK();
-    }
     const int64_t n = std::max<int64_t>(I * d, I);
     adjoint_finish_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(X, (int)I, (int)d, w.V, w.sw, w.r, G, dt,
                                                                       lambda1, (int)J, median, w.scal, grad_x,
