@@ -247,6 +247,23 @@ int wotk_transport_stablev2_coords(const double *X, const double *Y, int64_t I, 
                                    double *median_out, void *workspace, size_t workspace_bytes,
                                    wotk_stats *stats, void *stream);
 
+/* optimal_transport_duality_gap (wot/ot/optimal_transport.py:67-164) on the
+ * never-stored cost of wotk_transport_stablev2_coords: the same schedule as
+ * wotk_transport_duality_gap (epsilon scalings, dual-evolution checks, tau
+ * stabilisation counts, batches of batch_size on the last scaling, max_iter
+ * early return with un-normalised row sums, NaN gap -> WOTK_ENAN), every sweep
+ * regenerating C from X, Y on the matrix cores; the gap pass forms the plan's
+ * row and column sums with FP64 exponentials over the same regenerated C.
+ * Outputs as wotk_transport_stablev2_coords; stats carry iterations,
+ * stabilisations, gap checks, the final gap, primal and dual. */
+size_t wotk_duality_gap_coords_workspace_size(int64_t I, int64_t J, int64_t d);
+int wotk_transport_duality_gap_coords(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
+                                      const double *G, double lambda1, double lambda2, double epsilon,
+                                      int64_t batch_size, double tolerance, double tau, double epsilon0,
+                                      int64_t max_iter, int32_t terms, double *f_out, double *g_out,
+                                      double *row_sums, double *median_out, void *workspace,
+                                      size_t workspace_bytes, wotk_stats *stats, void *stream);
+
 /* Primal / dual objectives of optimal_transport.py:45-62 at the plan
  * R = exp((f_i + g_j - C_ij)/eps) (un-normalised), FP64 reductions.
  * out[0] = primal, out[1] = dual (device or host pointer to 2 doubles: host). */

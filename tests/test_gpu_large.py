@@ -139,3 +139,49 @@ def test_fused_nan_growth_propagates(mods):
     f = res.f.cpu().numpy()
     rs = res.row_sums.cpu().numpy()
     assert np.isnan(f[5]) and np.isnan(rs[5])
+
+
+GAP = dict(lambda1=1.0, lambda2=50.0, epsilon=0.05, batch_size=5, tolerance=1e-8, tau=1e4, epsilon0=1.0,
+           max_iter=1e7)
+
+
+@pytest.mark.parametrize('n_i,n_j,dim,seed', [(300, 360, 30, 14), (120, 150, 10, 15)])
+def test_fused_duality_gap_vs_oracle(mods, n_i, n_j, dim, seed):
+    """The duality-gap schedule on the never-stored cost (wot/ot/optimal_transport.py:67-164)
+    against the FP64 oracle on the reference cost: plan, marginals, converged
+    gap, and the iteration count within one batch."""
+    torch, ot, dev, large = mods
+    from oracle import ot_oracle as O
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(n_i, n_j, dim=dim, n_components=5, seed=seed)
+    st = {}
+    res = ot.optimal_transport_duality_gap(None, dp.growth, coords=(dp.x, dp.y), stats=st, **GAP)
+    R = res.dense().cpu().numpy()
+    so = {}
+    Ro = O.optimal_transport_duality_gap(O.default_cost_matrix(dp.x, dp.y), dp.growth, stats=so, **GAP)
+    err = np.linalg.norm(R - Ro) / np.linalg.norm(Ro)
+    marg = max(np.abs(R.sum(a) - Ro.sum(a)).max() / np.abs(Ro.sum(a)).max() for a in (0, 1))
+    print('fused duality gap vs oracle %dx%d: rel frob %.3e marginals %.3e, iterations %d vs %d, gap %.3e vs %.3e'
+          % (n_i, n_j, err, marg, st['iterations'], so['iterations'], st['gap'], so['gap']))
+    assert err <= 1e-5 and marg <= 1e-6
+    assert 0.0 <= st['gap'] <= GAP['tolerance']
+    assert abs(st['iterations'] - so['iterations']) <= GAP['batch_size']
+
+
+def test_fused_duality_gap_max_iter_early_return(mods, caplog):
+    """max_iter inside the dual-evolution scalings: the reference's warning and
+    its un-normalised plan (optimal_transport.py:143-145), against the oracle
+    stopped at the same iteration."""
+    torch, ot, dev, large = mods
+    import logging
+    from oracle import ot_oracle as O
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(90, 110, dim=12, n_components=4, seed=16)
+    prm = dict(GAP, max_iter=37)
+    with caplog.at_level(logging.WARNING, logger='wot'):
+        res = ot.optimal_transport_duality_gap(None, dp.growth, coords=(dp.x, dp.y), **prm)
+    assert 'Reached max_iter' in caplog.text
+    assert res.stats['max_iter_reached'] and res.stats['iterations'] == 37
+    R = res.dense().cpu().numpy()
+    Ro = O.optimal_transport_duality_gap(O.default_cost_matrix(dp.x, dp.y), dp.growth, **prm)
+    assert np.linalg.norm(R - Ro) / np.linalg.norm(Ro) <= 1e-5

@@ -103,6 +103,10 @@ SIGNATURES = {
     'wotk_transport_stablev2_coords': (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64, _D, _D, _I64,
                                                       _I64, _I32, _P, _P, _P, _P, _P, _SZ, ctypes.POINTER(Stats),
                                                       _P]),
+    'wotk_duality_gap_coords_workspace_size': (_SZ, [_I64, _I64, _I64]),
+    'wotk_transport_duality_gap_coords': (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64, _D, _D, _D,
+                                                         _I64, _I32, _P, _P, _P, _P, _P, _SZ, ctypes.POINTER(Stats),
+                                                         _P]),
     'wotk_gemm_f32': (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _I64, _I64, _I64, _P]),
     'wotk_adjoint_workspace_size': (_SZ, [_I64, _I64, _I64]),
     'wotk_ot_adjoint': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _I64, _P, _P, _I64, _P, _D, _P, _P, _D, _D, _D,

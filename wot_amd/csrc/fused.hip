@@ -19,7 +19,10 @@
 // maximum.  The median of the exact FP64 distances (the reference's
 // normalisation) comes from a radix select that regenerates the distances in
 // every pass.
+#include <type_traits>
+
 #include "common.h"
+#include "fused_problem.h"
 
 namespace wotk {
 
@@ -199,8 +202,9 @@ struct FusedArgs {
     double log_n_other, alpha, eps, k;
     int *bad_list;               // rows whose fixed-shift sum under/overflowed
     int *n_bad;
-    double *row_sums;            // row-sum mode: write sum_j R_ij / n_cols, leave pot unchanged
+    double *row_sums;            // row-sum mode: write sum_j R_ij / rs_div, leave pot unchanged
     double *part;                // column split (gridDim.y > 1): per-split row sums, split-major
+    double rs_div;               // row-sum mode divisor (n_cols: rows of R / J; 1: of R)
 };
 
 // Row epilogue of a sweep from the row's fixed-shift sum S: new potential
@@ -218,7 +222,7 @@ __device__ inline void fused_row_epilogue(const FusedArgs &fa, int row, double S
         return;
     }
     if (fa.row_sums) {  // the shift is the final potential: S = sum_j R_ij
-        fa.row_sums[row] = S / fa.n_cols;
+        fa.row_sums[row] = S / fa.rs_div;
         return;
     }
     const double lse = log(S) - f_old / fa.eps;  // ln sum_j exp((h_j - C_ij)/eps)
@@ -226,7 +230,10 @@ __device__ inline void fused_row_epilogue(const FusedArgs &fa, int row, double S
     fa.pot[row] = fa.alpha * fa.eps * (lw + fa.log_n_other - lse);
 }
 
-template <int TERMS, int NKS>
+// EXACT: FP64 exponential (exp2 of the FP64 exponent) and FP64 lane sums, for
+// the row and column sums of the duality-gap pass; otherwise v_exp_f32 of the
+// exponent rounded to FP32 and FP32 lane sums (the sweeps)
+template <int TERMS, int NKS, bool EXACT = false>
 __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
     constexpr int NP = TERMS == 3 ? 2 : 3;  // pieces per point
     constexpr int KP = NKS * 16;
@@ -241,7 +248,8 @@ __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
 
     // row constants for this lane's 16 accumulator rows: (f_old - |x'|^2) * k
     double wf[16];
-    float s[16];
+    using acc_t = typename std::conditional<EXACT, double, float>::type;
+    acc_t s[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int row = min(r0 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4), fa.n_rows - 1);
@@ -337,8 +345,13 @@ __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
         if (col < fa.n_cols) {
             const double uc = (h_cur - n_cur) * fa.k;
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                s[r] += __builtin_amdgcn_exp2f((float)fma(k2, (double)acc[r] + (double)acc2[r], uc + wf[r]));
+            for (int r = 0; r < 16; ++r) {
+                const double y = fma(k2, (double)acc[r] + (double)acc2[r], uc + wf[r]);
+                if constexpr (EXACT)
+                    s[r] += exp2(y);
+                else
+                    s[r] += __builtin_amdgcn_exp2f((float)y);
+            }
         }
         if (t + 1 < t_end) commit(buf ^ 1);  // the other buffer was released by the previous barrier
         h_cur = h_nxt;
@@ -383,7 +396,7 @@ __global__ __launch_bounds__(256) void fused_repair_kernel(const int *__restrict
                                                            double *__restrict__ pot, const double *__restrict__ logw,
                                                            const double *__restrict__ logw_dev, double log_n_other,
                                                            double alpha, double eps, double k,
-                                                           double *__restrict__ row_sums) {
+                                                           double *__restrict__ row_sums, double rs_div) {
     __shared__ double sg_sh[4], sd_sh[4];
     const int nb = *n_bad;
     for (int q = blockIdx.x; q < nb; q += gridDim.x) {
@@ -423,7 +436,7 @@ __global__ __launch_bounds__(256) void fused_repair_kernel(const int *__restrict
             }
             const double lse = (g + log2(s)) * kLn2;
             if (row_sums) {
-                row_sums[row] = exp(pot[row] / eps + lse) / n_cols;
+                row_sums[row] = exp(pot[row] / eps + lse) / rs_div;
             } else {
                 const double lw = logw ? logw[row] : *logw_dev;
                 pot[row] = alpha * eps * (lw + log_n_other - lse);
@@ -457,38 +470,6 @@ __global__ void fz_zero_kernel(double *x, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = 0.0;
 }
 
-struct FzWork {
-    double *mean, *xs, *ys, *nx, *ny, *f, *g, *logp, *misc;
-    __bf16 *xp[3], *yp[3];
-    int *bad, *nbad;
-    double *part;
-    FzSelect *sel;
-    unsigned long long *hist;
-};
-
-static size_t fz_carve(Arena &ar, int64_t I, int64_t J, int64_t d, FzWork *w) {
-    const int64_t kp = round_up(d, 16);
-    w->mean = ar.take<double>(d);
-    w->xs = ar.take<double>((size_t)I * d);
-    w->ys = ar.take<double>((size_t)J * d);
-    w->nx = ar.take<double>(I);
-    w->ny = ar.take<double>(J);
-    w->f = ar.take<double>(I);
-    w->g = ar.take<double>(J);
-    w->logp = ar.take<double>(I);
-    w->misc = ar.take<double>(8);
-    for (int p = 0; p < 3; ++p) {
-        w->xp[p] = ar.take<__bf16>((size_t)I * kp);
-        w->yp[p] = ar.take<__bf16>((size_t)J * kp);
-    }
-    w->bad = ar.take<int>(std::max(I, J));
-    w->nbad = ar.take<int>(4);
-    w->part = ar.take<double>((size_t)kFusedMaxSplit * std::max(I, J));
-    w->sel = ar.take<FzSelect>(1);
-    w->hist = ar.take<unsigned long long>(2048);
-    return ar.off;
-}
-
 static const int kFzShift[6] = {53, 42, 31, 20, 10, 0};
 static const int kFzWidth[6] = {11, 11, 11, 11, 10, 10};
 
@@ -501,41 +482,172 @@ static int fused_splits(int n_rows, int n_cols) {
     return std::max(1, std::min(std::min(want, kFusedMaxSplit), ntiles / 8));
 }
 
-template <int TERMS>
+template <int TERMS, bool EXACT>
 static void launch_fused(const FusedArgs &fa, int n_split, hipStream_t s) {
     const dim3 grid((unsigned)((fa.n_rows + 127) / 128), (unsigned)n_split);
     switch (fa.kp / 16) {
-        case 1: fused_lse_kernel<TERMS, 1><<<grid, kFThreads, 0, s>>>(fa); break;
-        case 2: fused_lse_kernel<TERMS, 2><<<grid, kFThreads, 0, s>>>(fa); break;
-        case 3: fused_lse_kernel<TERMS, 3><<<grid, kFThreads, 0, s>>>(fa); break;
-        case 4: fused_lse_kernel<TERMS, 4><<<grid, kFThreads, 0, s>>>(fa); break;
-        case 5: fused_lse_kernel<TERMS, 5><<<grid, kFThreads, 0, s>>>(fa); break;
-        case 6: fused_lse_kernel<TERMS, 6><<<grid, kFThreads, 0, s>>>(fa); break;
-        case 7: fused_lse_kernel<TERMS, 7><<<grid, kFThreads, 0, s>>>(fa); break;
-        default: fused_lse_kernel<TERMS, 8><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 1: fused_lse_kernel<TERMS, 1, EXACT><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 2: fused_lse_kernel<TERMS, 2, EXACT><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 3: fused_lse_kernel<TERMS, 3, EXACT><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 4: fused_lse_kernel<TERMS, 4, EXACT><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 5: fused_lse_kernel<TERMS, 5, EXACT><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 6: fused_lse_kernel<TERMS, 6, EXACT><<<grid, kFThreads, 0, s>>>(fa); break;
+        case 7: fused_lse_kernel<TERMS, 7, EXACT><<<grid, kFThreads, 0, s>>>(fa); break;
+        default: fused_lse_kernel<TERMS, 8, EXACT><<<grid, kFThreads, 0, s>>>(fa); break;
     }
 }
 
-template <int TERMS>
+template <int TERMS, bool EXACT = false>
 static int fused_sweep(const FusedArgs &fa, const double *xr, const double *xc, int d, hipStream_t s) {
     fused_clear_kernel<<<1, 1, 0, s>>>(fa.n_bad);
     const int n_split = fused_splits(fa.n_rows, fa.n_cols);
-    launch_fused<TERMS>(fa, n_split, s);
+    launch_fused<TERMS, EXACT>(fa, n_split, s);
     if (n_split > 1) fused_finalize_kernel<<<(fa.n_rows + 255) / 256, 256, 0, s>>>(fa, n_split);
     fused_repair_kernel<<<64, 256, 0, s>>>(fa.bad_list, fa.n_bad, xr, xc, fa.n_cols, d, fa.h, fa.pot, fa.logw,
-                                           fa.logw_dev, fa.log_n_other, fa.alpha, fa.eps, fa.k, fa.row_sums);
+                                           fa.logw_dev, fa.log_n_other, fa.alpha, fa.eps, fa.k, fa.row_sums,
+                                           fa.rs_div);
     WOTK_LAUNCH_CHECK();
     return WOTK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the problem helpers shared with the duality-gap schedule
+// ---------------------------------------------------------------------------
+static void problem_carve(Arena &ar, int64_t I, int64_t J, int64_t d, FusedProblem *fp) {
+    const int64_t kp = round_up(d, 16);
+    fp->mean = ar.take<double>(d);
+    fp->xs = ar.take<double>((size_t)I * d);
+    fp->ys = ar.take<double>((size_t)J * d);
+    fp->nx = ar.take<double>(I);
+    fp->ny = ar.take<double>(J);
+    fp->zeros = ar.take<double>(std::max(I, J));
+    for (int p = 0; p < 3; ++p) {
+        fp->xp[p] = ar.take<__bf16>((size_t)I * kp);
+        fp->yp[p] = ar.take<__bf16>((size_t)J * kp);
+    }
+    fp->bad = ar.take<int>(std::max(I, J));
+    fp->nbad = ar.take<int>(4);
+    fp->bad2 = ar.take<int>(std::max(I, J));
+    fp->nbad2 = ar.take<int>(4);
+    fp->part = ar.take<double>((size_t)kFusedMaxSplit * std::max(I, J));
+    fp->part2 = ar.take<double>((size_t)kFusedMaxSplit * std::max(I, J));
+    fp->sel = ar.take<FzSelect>(1);
+    fp->hist = ar.take<unsigned long long>(2048);
+}
+
+size_t fused_problem_bytes(int64_t I, int64_t J, int64_t d) {
+    Arena ar(nullptr, 0);
+    FusedProblem fp;
+    problem_carve(ar, I, J, d, &fp);
+    return ar.off;
+}
+
+int fused_problem_setup(const double *X, const double *Y, int64_t I, int64_t J, int64_t d, int terms,
+                        void *workspace, size_t workspace_bytes, FusedProblem *fp, size_t *used, hipStream_t s) {
+    WOTK_REQUIRE(I > 0 && J > 0 && d > 0 && I < (1ll << 30) && J < (1ll << 30), "bad shape");
+    WOTK_REQUIRE(d <= 128, "the fused solver keeps a 128-dimensional row tile in registers: d <= 128");
+    WOTK_REQUIRE(terms == 3 || terms == 6, "terms must be 3 or 6");
+    WOTK_REQUIRE(X && Y, "coordinates are required");
+    Arena ar(workspace, workspace_bytes);
+    problem_carve(ar, I, J, d, fp);
+    if (!workspace || !ar.ok) return fail(WOTK_EWORKSPACE, "fused workspace too small");
+    *used = ar.off;
+    fp->X = X;
+    fp->Y = Y;
+    fp->I = (int)I;
+    fp->J = (int)J;
+    fp->d = (int)d;
+    fp->kp = (int)round_up(d, 16);
+    fp->terms = terms;
+
+    // exact median of the FP64 distances, regenerated per pass
+    const long long n = (long long)I * J;
+    const long long rank = (n % 2) ? n / 2 : n / 2 - 1;
+    fz_select_init_kernel<<<1, 256, 0, s>>>(fp->sel, rank, fp->hist);
+    dim3 sgrid((unsigned)((J + 63) / 64), (unsigned)((I + 63) / 64));
+    for (int p = 0; p < 6; ++p) {
+        fz_select_kernel<0><<<sgrid, 256, 0, s>>>(X, Y, (int)I, (int)J, (int)d, kFzShift[p], kFzWidth[p], fp->sel,
+                                                  fp->hist);
+        fz_select_scan_kernel<<<1, 256, 0, s>>>(fp->hist, kFzShift[p], kFzWidth[p], p == 5, fp->sel);
+    }
+    if (n % 2 == 0) fz_select_kernel<1><<<sgrid, 256, 0, s>>>(X, Y, (int)I, (int)J, (int)d, 0, 0, fp->sel, fp->hist);
+    fz_select_finish_kernel<<<1, 1, 0, s>>>(fp->sel, (int)(n % 2));
+    WOTK_LAUNCH_CHECK();
+    fp->median = &fp->sel->median;
+
+    // scaled, centred coordinates and their bf16 pieces
+    fz_mean_kernel<<<(unsigned)d, 256, 0, s>>>(X, (int)I, Y, (int)J, (int)d, fp->mean);
+    const int np = terms == 3 ? 2 : 3;
+    __bf16 *const *xp = fp->xp, *const *yp = fp->yp;
+    fz_split_kernel<<<(unsigned)((I + 255) / 256), 256, 0, s>>>(X, (int)I, (int)d, fp->kp, fp->mean, fp->median, np,
+                                                                xp[0], xp[1], xp[2], fp->xs, fp->nx);
+    fz_split_kernel<<<(unsigned)((J + 255) / 256), 256, 0, s>>>(Y, (int)J, (int)d, fp->kp, fp->mean, fp->median, np,
+                                                                yp[0], yp[1], yp[2], fp->ys, fp->ny);
+    fz_zero_kernel<<<64, 256, 0, s>>>(fp->zeros, (int)std::max(I, J));
+    WOTK_LAUNCH_CHECK();
+    return WOTK_OK;
+}
+
+// FusedArgs of a sweep over source rows (over_targets = false) or target rows
+static FusedArgs problem_args(const FusedProblem &fp, bool target_rows, const double *h, double *pot,
+                              const double *logw, const double *logw_dev, double alpha, double eps, double *row_sums,
+                              double rs_div, bool second_list) {
+    __bf16 *const *A = target_rows ? fp.yp : fp.xp;
+    __bf16 *const *B = target_rows ? fp.xp : fp.yp;
+    const int nr = target_rows ? fp.J : fp.I, nc = target_rows ? fp.I : fp.J;
+    return FusedArgs{A[0], A[1], A[2], B[0], B[1], B[2], target_rows ? fp.ny : fp.nx, target_rows ? fp.nx : fp.ny,
+                     nr, nc, fp.kp, h, pot, logw, logw_dev, std::log((double)nc), alpha, eps, kLog2E / eps,
+                     second_list ? fp.bad2 : fp.bad, second_list ? fp.nbad2 : fp.nbad, row_sums,
+                     second_list ? fp.part2 : fp.part, rs_div};
+}
+
+template <bool EXACT>
+static int problem_sweep(const FusedProblem &fp, const FusedArgs &fa, bool target_rows, hipStream_t s) {
+    const double *xr = target_rows ? fp.ys : fp.xs, *xc = target_rows ? fp.xs : fp.ys;
+    return fp.terms == 3 ? fused_sweep<3, EXACT>(fa, xr, xc, fp.d, s) : fused_sweep<6, EXACT>(fa, xr, xc, fp.d, s);
+}
+
+int fused_problem_iterate(const FusedProblem &fp, double *f, double *g, const double *logp,
+                          const double *logq_dev, double lambda1, double lambda2, double eps, hipStream_t s) {
+    const FusedArgs fr = problem_args(fp, false, g, f, logp, nullptr, lambda1 / (lambda1 + eps), eps, nullptr, 1.0,
+                                      false);
+    int rc = problem_sweep<false>(fp, fr, false, s);
+    if (rc) return rc;
+    const FusedArgs fc = problem_args(fp, true, f, g, nullptr, logq_dev, lambda2 / (lambda2 + eps), eps, nullptr, 1.0,
+                                      false);
+    return problem_sweep<false>(fp, fc, true, s);
+}
+
+int fused_problem_sums(const FusedProblem &fp, bool over_targets, const double *f, const double *g, double eps,
+                       double div, bool exact, double *out, hipStream_t s) {
+    // row-sum mode: the shift is the row's own potential, which is not updated
+    const double *h = over_targets ? f : g;
+    double *pot = const_cast<double *>(over_targets ? g : f);
+    const FusedArgs fa = problem_args(fp, over_targets, h, pot, nullptr, nullptr, 0.0, eps, out, div, true);
+    return exact ? problem_sweep<true>(fp, fa, over_targets, s) : problem_sweep<false>(fp, fa, over_targets, s);
 }
 
 }  // namespace wotk
 
 using namespace wotk;
 
+// solver state of the fixed-iteration schedule beside the problem buffers
+struct FzState {
+    double *f, *g, *logp, *misc;
+};
+
+static size_t fz_state_carve(Arena &ar, int64_t I, int64_t J, FzState *st) {
+    st->f = ar.take<double>(I);
+    st->g = ar.take<double>(J);
+    st->logp = ar.take<double>(I);
+    st->misc = ar.take<double>(8);
+    return ar.off;
+}
+
 extern "C" size_t wotk_fused_workspace_size(int64_t I, int64_t J, int64_t d) {
     Arena ar(nullptr, 0);
-    FzWork w;
-    return fz_carve(ar, I, J, d, &w);
+    FzState st;
+    return fused_problem_bytes(I, J, d) + 256 + fz_state_carve(ar, I, J, &st);
 }
 
 extern "C" int wotk_transport_stablev2_coords(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
@@ -545,72 +657,48 @@ extern "C" int wotk_transport_stablev2_coords(const double *X, const double *Y, 
                                               double *row_sums, double *median_out, void *workspace,
                                               size_t workspace_bytes, wotk_stats *stats, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    WOTK_REQUIRE(I > 0 && J > 0 && d > 0 && I < (1ll << 30) && J < (1ll << 30), "bad shape");
-    WOTK_REQUIRE(d <= 128, "the fused solver keeps a 128-dimensional row tile in registers: d <= 128");
-    WOTK_REQUIRE(terms == 3 || terms == 6, "terms must be 3 or 6");
     WOTK_REQUIRE(scaling_iter >= 0 && extra_iter >= 0 && scaling_iter + extra_iter > 0 && inner_iter_max > 0,
                  "bad iteration counts");
+    WOTK_REQUIRE(G != nullptr, "G is required");
     (void)tau;
-    Arena ar(workspace, workspace_bytes);
-    FzWork w;
-    fz_carve(ar, I, J, d, &w);
-    if (!workspace || !ar.ok) return fail(WOTK_EWORKSPACE, "fused workspace too small");
-    const int kp = (int)round_up(d, 16);
-
-    // exact median of the FP64 distances, regenerated per pass
-    const long long n = (long long)I * J;
-    const long long rank = (n % 2) ? n / 2 : n / 2 - 1;
-    fz_select_init_kernel<<<1, 256, 0, s>>>(w.sel, rank, w.hist);
-    dim3 sgrid((unsigned)((J + 63) / 64), (unsigned)((I + 63) / 64));
-    for (int p = 0; p < 6; ++p) {
-        fz_select_kernel<0><<<sgrid, 256, 0, s>>>(X, Y, (int)I, (int)J, (int)d, kFzShift[p], kFzWidth[p], w.sel, w.hist);
-        fz_select_scan_kernel<<<1, 256, 0, s>>>(w.hist, kFzShift[p], kFzWidth[p], p == 5, w.sel);
-    }
-    if (n % 2 == 0) fz_select_kernel<1><<<sgrid, 256, 0, s>>>(X, Y, (int)I, (int)J, (int)d, 0, 0, w.sel, w.hist);
-    fz_select_finish_kernel<<<1, 1, 0, s>>>(w.sel, (int)(n % 2));
-    WOTK_LAUNCH_CHECK();
-    const double *med = &w.sel->median;
-    if (median_out) WOTK_HIP(hipMemcpyAsync(median_out, med, sizeof(double), hipMemcpyDeviceToDevice, s));
-
-    // scaled, centred coordinates and their bf16 pieces
-    fz_mean_kernel<<<(unsigned)d, 256, 0, s>>>(X, (int)I, Y, (int)J, (int)d, w.mean);
-    const int np = terms == 3 ? 2 : 3;
-    fz_split_kernel<<<(unsigned)((I + 255) / 256), 256, 0, s>>>(X, (int)I, (int)d, kp, w.mean, med, np, w.xp[0],
-                                                                w.xp[1], w.xp[2], w.xs, w.nx);
-    fz_split_kernel<<<(unsigned)((J + 255) / 256), 256, 0, s>>>(Y, (int)J, (int)d, kp, w.mean, med, np, w.yp[0],
-                                                                w.yp[1], w.yp[2], w.ys, w.ny);
+    FusedProblem fp;
+    size_t used = 0;
+    int rc = fused_problem_setup(X, Y, I, J, d, terms, workspace, workspace_bytes, &fp, &used, s);
+    if (rc) return rc;
+    used = (used + 255) & ~size_t(255);
+    Arena ar(static_cast<char *>(workspace) + used, workspace_bytes > used ? workspace_bytes - used : 0);
+    FzState w;
+    fz_state_carve(ar, I, J, &w);
+    if (!ar.ok) return fail(WOTK_EWORKSPACE, "fused workspace too small");
+    if (median_out) WOTK_HIP(hipMemcpyAsync(median_out, fp.median, sizeof(double), hipMemcpyDeviceToDevice, s));
     fz_zero_kernel<<<64, 256, 0, s>>>(w.f, (int)I);
     fz_zero_kernel<<<64, 256, 0, s>>>(w.g, (int)J);
     fz_growth_kernel<<<1, 1024, 0, s>>>(G, (int)I, w.logp, w.misc);
     WOTK_LAUNCH_CHECK();
 
     auto reg = [&](int64_t m) { return (epsilon0 - epsilon) * std::exp(-(double)m) + epsilon; };
-    const double lnJ = std::log((double)J), lnI = std::log((double)I);
-    hipEvent_t ev[2];
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    struct Events {
+        hipEvent_t *e;
+        ~Events() {
+            for (int i = 0; i < 2; ++i)
+                if (e[i]) (void)hipEventDestroy(e[i]);
+        }
+    } guard{ev};
     WOTK_HIP(hipEventCreate(&ev[0]));
     WOTK_HIP(hipEventCreate(&ev[1]));
     WOTK_HIP(hipEventRecord(ev[0], s));
     const int64_t n_iter = scaling_iter + extra_iter;
-    double eps = epsilon0;
     for (int64_t it = 0; it < n_iter; ++it) {
-        eps = it < scaling_iter ? reg(it / inner_iter_max) : reg(scaling_iter / inner_iter_max);
-        const double k = kLog2E / eps;
-        FusedArgs fr{w.xp[0], w.xp[1], w.xp[2], w.yp[0], w.yp[1], w.yp[2], w.nx, w.ny, (int)I, (int)J, kp,
-                     w.g, w.f, w.logp, nullptr, lnJ, lambda1 / (lambda1 + eps), eps, k, w.bad, w.nbad, nullptr, w.part};
-        FusedArgs fc{w.yp[0], w.yp[1], w.yp[2], w.xp[0], w.xp[1], w.xp[2], w.ny, w.nx, (int)J, (int)I, kp,
-                     w.f, w.g, nullptr, w.misc, lnI, lambda2 / (lambda2 + eps), eps, k, w.bad, w.nbad, nullptr, w.part};
-        int rc = terms == 3 ? fused_sweep<3>(fr, w.xs, w.ys, (int)d, s) : fused_sweep<6>(fr, w.xs, w.ys, (int)d, s);
-        if (rc) return rc;
-        rc = terms == 3 ? fused_sweep<3>(fc, w.ys, w.xs, (int)d, s) : fused_sweep<6>(fc, w.ys, w.xs, (int)d, s);
+        const double eps = it < scaling_iter ? reg(it / inner_iter_max) : reg(scaling_iter / inner_iter_max);
+        rc = fused_problem_iterate(fp, w.f, w.g, w.logp, w.misc, lambda1, lambda2, eps, s);
         if (rc) return rc;
     }
     WOTK_HIP(hipEventRecord(ev[1], s));
+    const double e_end = reg(scaling_iter / inner_iter_max);
     if (row_sums) {
         // the returned plan is built with the eps after the last iteration
-        const double e_end = reg(scaling_iter / inner_iter_max);
-        FusedArgs fr{w.xp[0], w.xp[1], w.xp[2], w.yp[0], w.yp[1], w.yp[2], w.nx, w.ny, (int)I, (int)J, kp,
-                     w.g, w.f, w.logp, nullptr, lnJ, 0.0, e_end, kLog2E / e_end, w.bad, w.nbad, row_sums, w.part};
-        int rc = terms == 3 ? fused_sweep<3>(fr, w.xs, w.ys, (int)d, s) : fused_sweep<6>(fr, w.xs, w.ys, (int)d, s);
+        rc = fused_problem_sums(fp, false, w.f, w.g, e_end, (double)J, false, row_sums, s);
         if (rc) return rc;
     }
     if (f_out) WOTK_HIP(hipMemcpyAsync(f_out, w.f, sizeof(double) * I, hipMemcpyDeviceToDevice, s));
@@ -618,14 +706,12 @@ extern "C" int wotk_transport_stablev2_coords(const double *X, const double *Y, 
     WOTK_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
     WOTK_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
     if (stats) {
         *stats = wotk_stats{};
         stats->iterations = n_iter;
         stats->sweeps = 2 * n_iter;
         stats->sweep_ms = ms;
-        stats->epsilon_final = reg(scaling_iter / inner_iter_max);
+        stats->epsilon_final = e_end;
     }
     return WOTK_OK;
 }

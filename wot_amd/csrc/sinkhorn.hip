@@ -22,6 +22,7 @@
 #include <mutex>
 
 #include "common.h"
+#include "fused_problem.h"
 
 namespace wotk {
 
@@ -814,21 +815,15 @@ struct SideStream {
     }
 };
 
-extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const float *CT,
-                                          int64_t ldct, int64_t I, int64_t J, const double *G,
-                                          double lambda1, double lambda2, double epsilon,
-                                          int64_t batch_size, double tolerance, double tau,
-                                          double epsilon0, int64_t max_iter, double *plan,
-                                          int64_t ldp, double *row_sums, double *f_out,
-                                          double *g_out, void *workspace, size_t workspace_bytes,
-                                          wotk_stats *stats, void *stream) {
-    hipStream_t s = (hipStream_t)stream;
-    Problem pb;
-    Work w;
-    int rc = setup(C, ldc, CT, ldct, I, J, G, workspace, workspace_bytes, &pb, &w, s);
-    if (rc) return rc;
+// The duality-gap schedule (optimal_transport.py:67-164) over a backend that
+// owns the cost: B::iterate (one (a, b) iteration on w.f, w.g), B::gap_pass
+// (primal, dual, gap at given potentials -> w.gapout), B::ktilde_sum and
+// B::finish (outputs at eps, plan divided by div).
+template <class B>
+static int gap_schedule(B &be, const Problem &pb, Work &w, double lambda1, double lambda2, double epsilon,
+                        int64_t batch_size, double tolerance, double tau, double epsilon0, int64_t max_iter,
+                        wotk_stats *stats, hipStream_t s) {
     WOTK_REQUIRE(batch_size > 0, "batch_size must be positive");
-    WOTK_REQUIRE(!plan || ldp >= J, "ldp < J");
     const int n_scal = 5;
     const double factor = std::exp(-std::log(epsilon) / n_scal);
     double eps_i = epsilon0 * factor;
@@ -837,9 +832,10 @@ extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const flo
     double host_out[3] = {0, 0, 0};
     double *sum_kt = w.misc + 4;
     double inv_eps = 1.0 / eps_i;
+    int rc = 0;
     // one (a, b) iteration plus the tau check (optimal_transport.py:131-141)
     auto one_iteration = [&](double eps) -> int {
-        int r = iterate(pb, w, lambda1, lambda2, eps, s);
+        int r = be.iterate(lambda1, lambda2, eps, s);
         if (r) return r;
         stabilise_kernel<<<1, 1024, 0, s>>>(w.f, w.u, pb.n_i, w.g, w.v, pb.n_j, inv_eps, tau, w.counter);
         WOTK_LAUNCH_CHECK();
@@ -847,7 +843,7 @@ extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const flo
     };
     // early, un-normalised return (optimal_transport.py:143-145)
     auto max_iter_return = [&](double eps) -> int {
-        int r = finish_outputs(pb, w, eps, 1.0, plan, ldp, row_sums, f_out, g_out, s);
+        int r = be.finish(eps, 1.0, s);
         if (r) return r;
         long long nst = 0;
         WOTK_HIP(hipMemcpyAsync(&nst, w.counter, sizeof(long long), hipMemcpyDeviceToHost, s));
@@ -871,7 +867,7 @@ extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const flo
         eps_i = eps_i / factor;
         inv_eps = 1.0 / eps_i;
         if (e == n_scal) {
-            rc = ktilde_sum(pb, w, eps_i, sum_kt, s);
+            rc = be.ktilde_sum(eps_i, sum_kt, s);
             if (rc) return rc;
         }
         if (e < n_scal) {
@@ -900,11 +896,11 @@ extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const flo
         }
         // Last scaling: a primal/dual gap pass after every batch of batch_size
         // iterations.  The pass (FP64 exp per entry, VALU-bound) runs on a side
-        // stream over a snapshot of the potentials while the next batch (HBM-bound
-        // sweeps) runs ahead on the caller's stream; when the gap has converged,
-        // the batch that ran ahead is rolled back from the snapshot, so the result
-        // is that of the sequential order.  No batch runs ahead when it would
-        // reach max_iter (its early return needs the state at max_iter).
+        // stream over a snapshot of the potentials while the next batch runs
+        // ahead on the caller's stream; when the gap has converged, the batch
+        // that ran ahead is rolled back from the snapshot, so the result is that
+        // of the sequential order.  No batch runs ahead when it would reach
+        // max_iter (its early return needs the state at max_iter).
         SideStream side;
         WOTK_HIP(hipStreamCreateWithFlags(&side.s, hipStreamNonBlocking));
         WOTK_HIP(hipEventCreateWithFlags(&side.snap, hipEventDisableTiming));
@@ -929,7 +925,7 @@ extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const flo
             WOTK_LAUNCH_CHECK();
             WOTK_HIP(hipEventRecord(side.snap, s));
             WOTK_HIP(hipStreamWaitEvent(side.s, side.snap, 0));
-            rc = gap_pass(pb, w, w.fs, w.gs, lambda1, lambda2, eps_i, sum_kt, side.s);
+            rc = be.gap_pass(w.fs, w.gs, lambda1, lambda2, eps_i, sum_kt, side.s);
             if (rc) return rc;
             if (it + nb < max_iter) {
                 for (int64_t t = 0; t < nb; ++t) {
@@ -963,10 +959,163 @@ extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const flo
     }
     if (std::isnan(gap))
         return fail(WOTK_ENAN, "Overflow encountered in duality gap computation, please report this incident");
-    rc = finish_outputs(pb, w, eps_i, (double)J, plan, ldp, row_sums, f_out, g_out, s);
+    rc = be.finish(eps_i, (double)pb.n_j, s);
     if (rc) return rc;
     WOTK_HIP(hipStreamSynchronize(s));
     return WOTK_OK;
+}
+
+// the stored FP32 cost (C, C^T) as the schedule's backend
+struct StoredBackend {
+    const Problem &pb;
+    Work &w;
+    double *plan;
+    int64_t ldp;
+    double *row_sums, *f_out, *g_out;
+    int iterate(double l1, double l2, double eps, hipStream_t s) { return wotk::iterate(pb, w, l1, l2, eps, s); }
+    int gap_pass(const double *f, const double *g, double l1, double l2, double eps, const double *skt,
+                 hipStream_t s) {
+        return wotk::gap_pass(pb, w, f, g, l1, l2, eps, skt, s);
+    }
+    int ktilde_sum(double eps, double *out, hipStream_t s) { return wotk::ktilde_sum(pb, w, eps, out, s); }
+    int finish(double eps, double div, hipStream_t s) {
+        return finish_outputs(pb, w, eps, div, plan, ldp, row_sums, f_out, g_out, s);
+    }
+};
+
+extern "C" int wotk_transport_duality_gap(const float *C, int64_t ldc, const float *CT,
+                                          int64_t ldct, int64_t I, int64_t J, const double *G,
+                                          double lambda1, double lambda2, double epsilon,
+                                          int64_t batch_size, double tolerance, double tau,
+                                          double epsilon0, int64_t max_iter, double *plan,
+                                          int64_t ldp, double *row_sums, double *f_out,
+                                          double *g_out, void *workspace, size_t workspace_bytes,
+                                          wotk_stats *stats, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    Problem pb;
+    Work w;
+    int rc = setup(C, ldc, CT, ldct, I, J, G, workspace, workspace_bytes, &pb, &w, s);
+    if (rc) return rc;
+    WOTK_REQUIRE(!plan || ldp >= J, "ldp < J");
+    StoredBackend be{pb, w, plan, ldp, row_sums, f_out, g_out};
+    return gap_schedule(be, pb, w, lambda1, lambda2, epsilon, batch_size, tolerance, tau, epsilon0, max_iter, stats,
+                        s);
+}
+
+// ---------------------------------------------------------------------------
+// The duality-gap schedule on the never-stored cost: every sweep regenerates
+// C from the coordinates (fused.hip), the gap pass forms the row and column
+// sums of R = exp((f_i + g_j - C_ij)/eps) with FP64 exponentials over the
+// same regenerated C, and the primal's entropy and transport terms come from
+// them: eps sum R log R + sum R C = sum_i f_i r_i + sum_j g_j c_j, since
+// log R_ij = (f_i + g_j - C_ij) / eps (an entry whose exp underflows adds 0
+// to both sides, as nan_to_num(log 0) * 0 does in the reference).
+
+// partial slots the gap finish reads: [0] = sum R log R, [1] = sum R C
+__global__ __launch_bounds__(1024) void gap_identity_kernel(const double *__restrict__ f, const double *__restrict__ r,
+                                                            int n_i, const double *__restrict__ g,
+                                                            const double *__restrict__ c, int n_j, double inv_eps,
+                                                            double *__restrict__ partials) {
+    __shared__ double red[16];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n_i; i += blockDim.x) acc += f[i] * r[i];
+    for (int j = threadIdx.x; j < n_j; j += blockDim.x) acc += g[j] * c[j];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+        partials[0] = t * inv_eps;
+        partials[1] = 0.0;
+        partials[2] = 0.0;
+    }
+}
+
+// sum of n doubles in a fixed order -> out[0]
+__global__ __launch_bounds__(1024) void sum_vec_kernel(const double *__restrict__ x, int n, double *__restrict__ out) {
+    __shared__ double red[16];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += x[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+        out[0] = t;
+    }
+}
+
+struct FusedBackend {
+    const Problem &pb;
+    Work &w;
+    const FusedProblem &fp;
+    double *row_sums, *f_out, *g_out;
+    int iterate(double l1, double l2, double eps, hipStream_t s) {
+        return fused_problem_iterate(fp, w.f, w.g, w.logp, w.misc, l1, l2, eps, s);
+    }
+    int gap_pass(const double *f, const double *g, double l1, double l2, double eps, const double *skt,
+                 hipStream_t s) {
+        int rc = fused_problem_sums(fp, false, f, g, eps, 1.0, true, w.r, s);
+        if (rc) return rc;
+        rc = fused_problem_sums(fp, true, f, g, eps, 1.0, true, w.c, s);
+        if (rc) return rc;
+        gap_identity_kernel<<<1, 1024, 0, s>>>(f, w.r, pb.n_i, g, w.c, pb.n_j, 1.0 / eps, w.partials);
+        GapArgs ga{w.r, w.c, f, g, pb.G, w.misc, w.partials, 1, pb.n_i, pb.n_j, l1, l2, eps, 0.0, skt, w.gapout};
+        gap_finish_kernel<<<1, 1024, 0, s>>>(ga);
+        WOTK_LAUNCH_CHECK();
+        return WOTK_OK;
+    }
+    int ktilde_sum(double eps, double *out, hipStream_t s) {
+        // row sums of exp(-C/eps): potentials zero, FP64 exponentials
+        int rc = fused_problem_sums(fp, false, fp.zeros, fp.zeros, eps, 1.0, true, w.r, s);
+        if (rc) return rc;
+        sum_vec_kernel<<<1, 1024, 0, s>>>(w.r, pb.n_i, out);
+        WOTK_LAUNCH_CHECK();
+        return WOTK_OK;
+    }
+    int finish(double eps, double div, hipStream_t s) {
+        if (row_sums) {
+            int rc = fused_problem_sums(fp, false, w.f, w.g, eps, div, false, row_sums, s);
+            if (rc) return rc;
+        }
+        if (f_out) WOTK_HIP(hipMemcpyAsync(f_out, w.f, sizeof(double) * pb.n_i, hipMemcpyDeviceToDevice, s));
+        if (g_out) WOTK_HIP(hipMemcpyAsync(g_out, w.g, sizeof(double) * pb.n_j, hipMemcpyDeviceToDevice, s));
+        return WOTK_OK;
+    }
+};
+
+extern "C" size_t wotk_duality_gap_coords_workspace_size(int64_t I, int64_t J, int64_t d) {
+    Arena ar(nullptr, 0);
+    Work w;
+    return fused_problem_bytes(I, J, d) + 256 + carve(ar, I, J, &w);
+}
+
+extern "C" int wotk_transport_duality_gap_coords(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
+                                                 const double *G, double lambda1, double lambda2, double epsilon,
+                                                 int64_t batch_size, double tolerance, double tau, double epsilon0,
+                                                 int64_t max_iter, int32_t terms, double *f_out, double *g_out,
+                                                 double *row_sums, double *median_out, void *workspace,
+                                                 size_t workspace_bytes, wotk_stats *stats, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    WOTK_REQUIRE(G != nullptr, "G is required");
+    FusedProblem fp;
+    size_t used = 0;
+    int rc = fused_problem_setup(X, Y, I, J, d, terms, workspace, workspace_bytes, &fp, &used, s);
+    if (rc) return rc;
+    used = (used + 255) & ~size_t(255);
+    Arena ar(static_cast<char *>(workspace) + used, workspace_bytes > used ? workspace_bytes - used : 0);
+    Work w;
+    carve(ar, I, J, &w);
+    if (!ar.ok) return fail(WOTK_EWORKSPACE, "workspace too small");
+    const Problem pb{nullptr, 0, nullptr, 0, (int)I, (int)J, G};
+    rc = init_state(pb, w, s);
+    if (rc) return rc;
+    if (median_out) WOTK_HIP(hipMemcpyAsync(median_out, fp.median, sizeof(double), hipMemcpyDeviceToDevice, s));
+    FusedBackend be{pb, w, fp, row_sums, f_out, g_out};
+    return gap_schedule(be, pb, w, lambda1, lambda2, epsilon, batch_size, tolerance, tau, epsilon0, max_iter, stats,
+                        s);
 }
 
 extern "C" int wotk_plan(const float *C, int64_t ldc, int64_t I, int64_t J, const double *f, const double *g,

@@ -42,13 +42,28 @@ def transport_stablev2(C, lambda1, lambda2, epsilon, scaling_iter, G, tau, epsil
 
 
 def optimal_transport_duality_gap(C, G, lambda1, lambda2, epsilon, batch_size, tolerance, tau, epsilon0,
-                                  max_iter, device_out=False, stats=None, **ignored):
+                                  max_iter, device_out=False, stats=None, coords=None, terms=6, **ignored):
     """Duality-gap solver, OTModel's default (wot/ot/optimal_transport.py:67-164).
 
     Five epsilon scalings down to eps0*eps, dual-evolution checks (1e-6) then
     a relative duality gap <= ``tolerance``; returns R / J, or the
     un-normalised plan with a warning when ``max_iter`` is reached; a NaN gap
-    raises RuntimeError."""
+    raises RuntimeError.
+
+    With ``C=None`` and ``coords=(x, y)`` the cost sqeuclid(x, y) / median is
+    never stored (split-BF16 cross term on the matrix cores with ``terms`` = 3
+    or 6 products, regenerated per sweep and per gap pass); the result is then
+    a ``large.CoordsResult`` holding the potentials."""
+    if C is None:
+        if coords is None:
+            raise ValueError('optimal_transport_duality_gap needs a cost matrix C or coords=(x, y)')
+        from . import large
+        res = large.optimal_transport_duality_gap_coords(coords[0], coords[1], G, lambda1=lambda1, lambda2=lambda2,
+                                                         epsilon=epsilon, batch_size=batch_size, tolerance=tolerance,
+                                                         tau=tau, epsilon0=epsilon0, max_iter=max_iter, terms=terms)
+        if stats is not None:
+            stats.update(res.stats)
+        return res
     cost = _dev.ensure_cost(C)
     res = _dev.solve(cost, G, 'duality_gap',
                      dict(lambda1=lambda1, lambda2=lambda2, epsilon=epsilon, batch_size=batch_size,
