@@ -682,6 +682,19 @@ def bench_large(args):
     out['value'] = out['terms3']['maps_per_s']
     out['ms_per_step'] = 1000.0 * out['terms3']['seconds']
     out['sinkhorn_iters_per_s'] = out['terms3']['sinkhorn_iters_per_s']
+    # OTModel's default solver on the same pair, cost never stored (6 terms)
+    gap_prm = dict(lambda1=SOLVER['lambda1'], lambda2=SOLVER['lambda2'], epsilon=SOLVER['epsilon'], batch_size=5,
+                   tolerance=1e-8, tau=1e4, epsilon0=SOLVER['epsilon0'], max_iter=1e7)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rg = large.optimal_transport_duality_gap_coords(X, Y, G, terms=6, **gap_prm)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    out['duality_gap_terms6'] = {'seconds': el, 'maps_per_s': 1.0 / el, 'iterations': rg.stats['iterations'],
+                                 'gap_checks': rg.stats['gap_checks'], 'gap': rg.stats['gap'],
+                                 'stabilisations': rg.stats['stabilisations']}
+    log('duality gap (6 terms): %.2f s, %d iterations' % (el, rg.stats['iterations']))
+    del rg
     # full-size cross-check on a short schedule against the stored-cost solver
     short = dict(sched, scaling_iter=100, extra_iter=0)
     cost = ot.DeviceCost.from_coordinates(X, Y)
@@ -1164,6 +1177,40 @@ def bench_validation(args):
     return out
 
 
+def bench_sweeps(args):
+    """Short, fixed workload for counter and trace runs of the stored-cost
+    solver: the 15,000 x 15,000 pair's cost (``--cost-reps`` extra builds in
+    each cost mode first), ``--iters`` fixed-iteration Sinkhorn iterations (2
+    row log-sum-exp sweeps each, one over C and one over C^T, 4*n*n algorithmic
+    bytes per sweep) with the given ``--tile``, then one plain device copy of
+    n*n float32 (n*n*4 bytes read and written) that calibrates the FETCH_SIZE
+    and WRITE_SIZE counters."""
+    import torch
+    import wot_amd.ot as ot
+    from wot_amd.synthetic import day_pair
+    from wot_amd import tune
+    n = CFG['n_source']
+    iters = args.iters if args.iters is not None else 20
+    dp = day_pair(n, n, dim=CFG['dim'], n_components=CFG['n_components'], seed=CFG['seed'])
+    if args.tile:
+        tune.apply_persisted()
+        tune.set_tile(tune.ROW_SWEEP, -1, _parse_tile(args.tile))
+    for _ in range(args.cost_reps):
+        for mode in ('f64', 'bf16x3', 'bf16x6'):
+            ot.DeviceCost.from_coordinates(dp.x, dp.y, mode=mode)
+    cost = ot.DeviceCost.from_coordinates(dp.x, dp.y)
+    st = {}
+    ot.transport_stablev2(cost, G=dp.growth, lambda1=1.0, lambda2=50.0, epsilon=0.05, scaling_iter=iters, tau=1e4,
+                          epsilon0=1.0, extra_iter=0, inner_iter_max=50, device_out=True, stats=st)
+    src = torch.empty((n, n), dtype=torch.float32, device='cuda').fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    return {'workload': 'row sweeps of one %d x %d pair' % (n, n), 'iterations': iters, 'tile': args.tile,
+            'avg_sweep_us': 1000.0 * st['sweep_ms'] / max(st['sweeps'], 1), 'copy_bytes': src.numel() * 4}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -1173,13 +1220,15 @@ def main():
     ap.add_argument('--large-n', type=int, default=60000, help='large only: cells per day')
     ap.add_argument('--group-size', type=int, default=None,
                     help='timecourse only: combinations per batched run (default: the whole grid)')
-    ap.add_argument('--config', default='timecourse', choices=['pair', 'gap', 'timecourse', 'adjoint', 'large', 'tmap', 'pca', 'validation'],
+    ap.add_argument('--config', default='timecourse', choices=['pair', 'gap', 'timecourse', 'adjoint', 'large', 'tmap', 'pca',
+                                                               'validation', 'sweeps'],
                     help='timecourse: 38 pairs x 9-point grid (default); pair: one 15k x 15k map; '
                          'gap: the same pair with the duality-gap solver; '
                          'adjoint: gradient of the converged objective, 8k x 8k; '
                          'tmap: trajectories through the 38 maps of the time course; '
                          'pca: local PCA of one 15k + 15k cell day pair; '
-                         'validation: compute_validation_summary on one day triplet')
+                         'validation: compute_validation_summary on one day triplet; '
+                         'sweeps: a short row-sweep workload for counter runs')
     ap.add_argument('--tmap-dtype', default='float64', choices=['float64', 'float32'],
                     help='tmap only: precision of the stored maps')
     ap.add_argument('--cost-mode', default='bf16x6', choices=['f64', 'bf16x3', 'bf16x6'],
@@ -1192,6 +1241,8 @@ def main():
                     help='timecourse: force the shared-exponential sweep tile ROWSxTHREADS (no tuning)')
     ap.add_argument('--no-parity', action='store_true',
                     help='timecourse: skip the parity and cost-error legs (profiling runs)')
+    ap.add_argument('--cost-reps', type=int, default=0,
+                    help='sweeps: extra cost builds per cost mode before the sweeps (kernel traces)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: libwotk has no multi-GPU path (got --gpus %d)' % args.gpus,
@@ -1217,6 +1268,9 @@ def main():
         return
     if args.config == 'validation':
         print(json.dumps(bench_validation(args)), flush=True)
+        return
+    if args.config == 'sweeps':
+        print(json.dumps(bench_sweeps(args)), flush=True)
         return
 
     import torch

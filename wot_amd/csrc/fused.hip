@@ -9,10 +9,11 @@
 //   3 terms: hi.hi + hi.lo' + lo'.hi  (lo' = bf16(x' - hi); ~2^-16 relative)
 //   6 terms: hi.hi + hi.mid + mid.hi + hi.lo + lo.hi + mid.mid (~2^-24 relative)
 // accumulated in FP32 (v_mfma_f32_32x32x16_bf16) in two chains, hi.hi and the
-// smaller products, which are added in FP64 (their FP32 sum would round at
-// 2^-24 of the cross term; in FP64 the marginals of a 97 x 131, d = 7 pair at
-// eps = 0.05 move from 1.6e-6 to 6.7e-7 of the FP64 oracle's, for 6 % more
-// sweep time at 60k).  Per entry the exponent is
+// smaller products.  With 6 terms they are added in FP64 (their FP32 sum would
+// round at 2^-24 of the cross term; in FP64 the marginals of a 97 x 131, d = 7
+// pair at eps = 0.05 move from 1.6e-6 to 6.7e-7 of the FP64 oracle's, for 6 %
+// more sweep time at 60k); with 3 terms, whose missing products already cost
+// 2^-16, in FP32.  Per entry the exponent is
 // formed in FP64 against the row's previous potential (a fixed shift, as in
 // the batched sweep); rows whose sum under- or overflows are queued and
 // recomputed by a repair kernel from FP64 direct distances with an online
@@ -346,7 +347,10 @@ __global__ __launch_bounds__(kFThreads) void fused_lse_kernel(FusedArgs fa) {
             const double uc = (h_cur - n_cur) * fa.k;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const double y = fma(k2, (double)acc[r] + (double)acc2[r], uc + wf[r]);
+                // 6 terms: the chains added in FP64; 3 terms (2^-16 of the cross term already
+                // lost to the missing products): one FP32 add and one widening
+                const double xy = TERMS == 6 ? (double)acc[r] + (double)acc2[r] : (double)(acc[r] + acc2[r]);
+                const double y = fma(k2, xy, uc + wf[r]);
                 if constexpr (EXACT)
                     s[r] += exp2(y);
                 else
