@@ -262,7 +262,8 @@ def bench_timecourse(args):
     combos, eps_a, l1_a, l2_a = batch.grid_arrays(len(pair_ij), GRID['epsilons'], GRID['lambda1s'],
                                                   sched['lambda2'])
     run_kw = dict(scaling_iter=sched['scaling_iter'], epsilon0=sched['epsilon0'], extra_iter=sched['extra_iter'],
-                  inner_iter_max=sched['inner_iter_max'], graph_iters=50, combos=combos)
+                  inner_iter_max=sched['inner_iter_max'], graph_iters=50, combos=combos,
+                  share_eps=args.share_eps == 'on')
     tiles = None
     if args.tile:
         from wot_amd import tune
@@ -289,7 +290,7 @@ def bench_timecourse(args):
         torch.cuda.empty_cache()
     torch.cuda.synchronize()
     plan_buf = torch.empty(max(pair_ij), dtype=torch.float64, device='cuda')
-    sweep_ms, sweeps, exact_rows, shared = 0.0, 0, 0, None
+    sweep_ms, sweeps, exact_rows, shared, derived_iters = 0.0, 0, 0, None, None
     res = tc = run = None
     courses = 0
     t0 = time.perf_counter()
@@ -300,6 +301,7 @@ def bench_timecourse(args):
         left = run.advance(step_iters)
         if left == 0:
             res = run.result()
+            derived_iters = int((run.modes() != 0).sum())
             for p in range(len(pair_ij)):
                 n_i, n_j = res.shapes[p]
                 for e in range(n_combo):
@@ -348,6 +350,8 @@ def bench_timecourse(args):
         'sinkhorn_iters_per_s': n_solves * args.steps * step_iters / elapsed,
         'sweep_loop_s_per_course': (sweep_ms / 1000.0) / max(courses, 1) if courses else None,
         'shared_exponential_group': shared,
+        'share_eps': args.share_eps,
+        'iterations_with_derived_eps_groups': derived_iters,
         'rows_recomputed_exactly': exact_rows,
         'roofline': {'bound': 'hbm', 'kernel': _tc_kernel_name({'shared_group': shared}, tiles),
                      'avg_launch_us': avg_sweep * 1e6, 'alg_bytes_per_launch': alg_bytes,
@@ -1241,6 +1245,9 @@ def main():
                     help='timecourse: force the shared-exponential sweep tile ROWSxTHREADS (no tuning)')
     ap.add_argument('--no-parity', action='store_true',
                     help='timecourse: skip the parity and cost-error legs (profiling runs)')
+    ap.add_argument('--share-eps', default='on', choices=['on', 'off'],
+                    help='timecourse: eps groups in integer k ratios take powers of the largest eps\'s '
+                         'exponential once the schedule has converged')
     ap.add_argument('--cost-reps', type=int, default=0,
                     help='sweeps: extra cost builds per cost mode before the sweeps (kernel traces)')
     args = ap.parse_args()

@@ -175,12 +175,20 @@ int wotk_plan(const float *C, int64_t ldc, int64_t I, int64_t J, const double *f
  * whose factored sums could have lost mass to FP32 underflow, or overflowed,
  * are recomputed exactly.  Ignored (the per-combination kernel runs) when the
  * grid has no such grouping.
+ * flags & WOTK_BATCH_SHARE_EPS (with WOTK_BATCH_SHARE_EXP, three groups of
+ * three): at every iteration where k = log2(e)/eps of groups 0 and 1 is an
+ * integer multiple m of the last group's (within 1e-10 relative, for every
+ * pair) -- e.g. eps {0.01, 0.05, 0.1} once the eps schedule has converged --
+ * those groups take the last group's exponential to the power m (FP32
+ * products) instead of their own; compiled powers (group 1, group 0): (2, 10),
+ * (2, 4), (2, own).  wotk_batch_modes reports the powers per iteration.
  * stats: sweeps / sweep_ms as for the single solver; stabilisations = rows
  * recomputed exactly (fixed shift or factors out of range); gap_checks =
  * combinations per shared-exponential group (0: one exponential per
  * combination).
  * ------------------------------------------------------------------------- */
 #define WOTK_BATCH_SHARE_EXP 1
+#define WOTK_BATCH_SHARE_EPS 2
 
 typedef struct wotk_batch_pair {
     int64_t c_offset;   /* element offset of C_p (I_p x J_p, ldc) in the cost buffer */
@@ -227,6 +235,11 @@ int wotk_batch_begin(const float *cost, const wotk_batch_pair *pairs, int32_t n_
 int wotk_batch_advance(wotk_batch_run *run, int64_t n_iterations, int64_t *remaining, void *stream);
 int wotk_batch_potentials(wotk_batch_run *run, double *f_out, double *g_out, double *eps_final,
                           wotk_stats *stats, void *stream);
+/* Per iteration of the run's schedule (n >= its iteration count, HOST
+ * array): the powers groups 1 (bits 0-7) and 0 (bits 8-15) take of the last
+ * group's exponential under WOTK_BATCH_SHARE_EPS, 0 where each group uses its
+ * own.  No counterpart in the reference (it solves each combination alone). */
+int wotk_batch_modes(const wotk_batch_run *run, int32_t *modes, int64_t n);
 void wotk_batch_end(wotk_batch_run *run);
 
 /* ---------------------------------------------------------------------------

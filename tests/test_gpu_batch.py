@@ -135,3 +135,74 @@ def test_batch_single_combo_equals_single_solver(batch):
     R1 = ot.transport_stablev2(cost, G=growth[0], lambda1=1.0, lambda2=50.0, epsilon=0.05, epsilon0=1.0, **SCHED)
     R2 = res.plan(0, 0)
     assert np.linalg.norm(R1 - R2) / np.linalg.norm(R1) < 1e-6
+
+
+# eps grids whose k = log2(e)/eps stand in integer ratios to the largest eps's:
+# (groups 1, 0) take its exponential squared and to the 10th / 4th power, or
+# group 0 keeps its own (ratio 10/3)
+@pytest.mark.parametrize('epsilons,mode', [
+    ((0.01, 0.05, 0.1), 2 | (10 << 8)),
+    ((0.025, 0.05, 0.1), 2 | (4 << 8)),
+    ((0.03, 0.05, 0.1), 2),
+])
+def test_shared_eps_powers_match_oracle(batch, epsilons, mode):
+    """With WOTK_BATCH_SHARE_EPS, once the eps schedule has converged (here
+    from iteration 140 of 400: eps changes every 5 iterations, and
+    |k_g / k_base - m| <= 1e-10 m from e^-28 on) the smaller-eps groups take
+    powers of the largest eps's exponential; every plan still meets the
+    oracle gate, and differs from the run with each group's own exponential
+    by far less than the gate."""
+    sched = dict(scaling_iter=300, extra_iter=100, inner_iter_max=5, tau=1e4)
+    coords, growth = _course(11, [150, 320, 200])
+    lams = (0.5, 1.0, 5.0)
+    combos, eps, l1, l2 = batch.grid_arrays(2, epsilons, lams, 50.0)
+    tc = batch.prepare_time_course(coords, growth)
+    with batch.TimeCourseRun(tc, eps, l1, l2, epsilon0=1.0, graph_iters=50, share_eps=True, combos=combos,
+                             **{k: v for k, v in sched.items() if k != 'tau'}) as run:
+        run.advance(run.n_iter)
+        modes = run.modes()
+        res = run.result()
+    # the host's choice per iteration: the powers whose ratios hold within 1e-10,
+    # among the compiled modes (group 0 falls back to its own exponential)
+    e0 = np.array(epsilons)
+    compiled = (2 | (10 << 8), 2 | (4 << 8), 2)
+    want = np.zeros(run.n_iter, dtype=np.int32)
+    for n in range(run.n_iter):
+        cur = (1.0 - e0) * np.exp(-float(min(n, 300) // 5)) + e0
+        r = cur[2] / cur[:2]
+        pw = [int(round(r[g])) if abs(r[g] - round(r[g])) <= 1e-10 * round(r[g]) and round(r[g]) >= 2 else 0
+              for g in (0, 1)]
+        full, part = pw[1] | (pw[0] << 8), pw[1]
+        want[n] = full if full in compiled else (part if part in compiled else 0)
+    assert np.array_equal(modes, want), (np.unique(modes), np.unique(want))
+    first = int(np.argmax(want != 0))
+    assert 100 < first < 200 and (want[200:] == mode).all()
+    worst = 0.0
+    for p in range(2):
+        C = O.default_cost_matrix(coords[p], coords[p + 1])
+        for e, (ep, lam1) in enumerate(combos):
+            R = res.plan(p, e)
+            Rref = O.transport_stablev2(C, lam1, 50.0, ep, G=growth[p], epsilon0=1.0, **sched)
+            err = np.linalg.norm(R - Rref) / np.linalg.norm(Rref)
+            marg = max(np.abs(R.sum(a) - Rref.sum(a)).max() / np.abs(Rref.sum(a)).max() for a in (0, 1))
+            worst = max(worst, err)
+            assert err <= 1e-5 and marg <= 1e-6, (p, ep, lam1, err, marg)
+    own = batch.solve_prepared(tc, epsilons, lams, epsilon0=1.0, graph_iters=50, share_eps=False, **sched)
+    diff = max(np.linalg.norm(res.plan(p, e) - own.plan(p, e)) / np.linalg.norm(own.plan(p, e))
+               for p in range(2) for e in range(9))
+    print('mode %d from iteration %d: worst vs oracle %.2e, vs own exponentials %.2e, exact rows %d'
+          % (mode, first, worst, diff, res.stats['exact_rows']))
+    assert diff <= 2e-6
+
+
+def test_shared_eps_needs_the_flag_and_converged_ratios(batch):
+    """Without the flag, or while the schedule's eps ratios are not integers,
+    every group keeps its own exponential."""
+    coords, growth = _course(12, [90, 130])
+    combos, eps, l1, l2 = batch.grid_arrays(1, (0.01, 0.05, 0.1), (0.5, 1.0, 5.0), 50.0)
+    tc = batch.prepare_time_course(coords, growth)
+    for share_eps, sched in ((False, dict(scaling_iter=300, extra_iter=100, inner_iter_max=5)),
+                             (True, dict(scaling_iter=300, extra_iter=100, inner_iter_max=50))):
+        with batch.TimeCourseRun(tc, eps, l1, l2, epsilon0=1.0, graph_iters=50, share_eps=share_eps,
+                                 combos=combos, **sched) as run:
+            assert (run.modes() == 0).all(), share_eps

@@ -24,6 +24,7 @@ KERNEL_ROW_SWEEP = 0
 KERNEL_BATCH_SWEEP = 1
 KERNEL_BATCH_SHARED = 2
 BATCH_SHARE_EXP = 1
+BATCH_SHARE_EPS = 2
 
 DTYPE_F64 = 0
 DTYPE_F32 = 1
@@ -98,6 +99,7 @@ SIGNATURES = {
                                         _P, _SZ, ctypes.POINTER(ctypes.c_void_p), _P]),
     'wotk_batch_advance': (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_int64), _P]),
     'wotk_batch_potentials': (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(Stats), _P]),
+    'wotk_batch_modes': (ctypes.c_int, [_P, _P, _I64]),
     'wotk_batch_end': (None, [_P]),
     'wotk_fused_workspace_size': (_SZ, [_I64, _I64, _I64]),
     'wotk_transport_stablev2_coords': (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _D, _D, _D, _I64, _D, _D, _I64,

@@ -47,7 +47,41 @@ struct IterParam {  // one (iteration, pair, combination)
     double k;       // log2(e) / eps
     double eps;
     double alpha1, alpha2;
+    int share;      // exponentials shared across eps groups at this iteration (see derived_power)
+    int pad_;
 };
+
+// Exponentials shared across eps groups.  With three eps groups whose k =
+// log2(e)/eps stand in integer ratios to the last (base) group's -- once the
+// schedule's eps has converged, e.g. eps {0.01, 0.05, 0.1}: k_0 = 10 k_2,
+// k_1 = 2 k_2 -- 2^(k_g y) = (2^(k_2 y))^m: a derived group g takes the
+// base's term to the power m instead of its own exponential, with column
+// factors relative to the base's reference potentials.  `mode` packs the
+// powers: group 1's in bits 0-7, group 0's in bits 8-15 (0: own exponential).
+__device__ __host__ __forceinline__ constexpr int derived_power(int mode, int g, int ng) {
+    return ng != 3 ? 0 : g == 1 ? (mode & 0xff) : g == 0 ? ((mode >> 8) & 0xff) : 0;
+}
+
+// t^m for 1 <= m < 32 by binary powering (constant m: the loop folds to the
+// squarings and products m needs; m = 10 is 4 FP32 products)
+__device__ __forceinline__ float powi(float t, int m) {
+    float r = 1.f;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+        if ((m >> b) & 1) r *= t;
+        if ((m >> (b + 1)) == 0) break;
+        t *= t;
+    }
+    return r;
+}
+
+// the combination whose potential combination e's column factors are relative
+// to, for a reading sweep in `mode`: its group's reference, or the base
+// group's for a derived group
+__device__ __forceinline__ int factor_ref(int mode, int e, int nl, int ng) {
+    const int g = e / nl;
+    return derived_power(mode, g, ng) ? (ng - 1) * nl + nl / 2 : g * nl + nl / 2;
+}
 
 struct Tile {
     int side_pair;  // index into the PairDev table of this side
@@ -92,14 +126,21 @@ __device__ __forceinline__ void queue_row(const BatchArgs &ba, int sp, int row, 
     ba.bad_list[3 * i + 2] = mask;
 }
 
-// a failing reference invalidates every factor of its group: widen the mask to
-// whole groups (NL combinations per group, reference NL / 2)
+// a failing reference invalidates every factor relative to it: widen the mask
+// to whole groups (NL combinations per group, reference NL / 2), and with the
+// base group's reference also to the groups the next sweep (in mode nmode)
+// derives from the base
 template <int NG, int NL>
-__device__ __forceinline__ int widen_to_groups(int m) {
+__device__ __forceinline__ int widen_to_groups(int m, int nmode = 0) {
     int w = m;
 #pragma unroll
     for (int g = 0; g < NG; ++g)
         if ((m >> (g * NL + NL / 2)) & 1) w |= ((1 << NL) - 1) << (g * NL);
+    if ((m >> ((NG - 1) * NL + NL / 2)) & 1) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (derived_power(nmode, g, NG)) w |= ((1 << NL) - 1) << (g * NL);
+    }
     return w;
 }
 
@@ -463,10 +504,11 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
         __syncthreads();
         // combinations out of range go to the repair kernel (whole groups when the
         // reference failed); no barrier sits under a condition
+        const int nmode = nxt[0].share;  // the reading sweep's mode: which reference each factor is relative to
         if (tid < RE) {
             const int r = tid / E, e = tid % E;
             const int row = tl.row0 + r;
-            const int m = widen_to_groups<NG, NL>(bad_row[r]);
+            const int m = widen_to_groups<NG, NL>(bad_row[r], nmode);
             const double p = new_potential(ba, pd, prm[e], min(row, pd.nrows - 1), tl.side_pair, l2_sh[tid], e);
             pot_sh[tid] = p;
             if (row < pd.nrows) {
@@ -479,10 +521,10 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
             // this side's factors, read by the other side's next sweep with k of that sweep
             const int r = tid / E, e = tid % E, g = e / NL;
             const int row = tl.row0 + r;
-            if (row < pd.nrows && !((widen_to_groups<NG, NL>(bad_row[r]) >> e) & 1)) {
+            if (row < pd.nrows && !((widen_to_groups<NG, NL>(bad_row[r], nmode) >> e) & 1)) {
                 float dpos;
                 ba.fac[pd.pot_off + (long long)e * pd.pot_ld + row] =
-                    factor_of(nxt[g * NL + L0].k, pot_sh[tid], pot_sh[r * E + g * NL + L0], &dpos);
+                    factor_of(nxt[g * NL + L0].k, pot_sh[tid], pot_sh[r * E + factor_ref(nmode, e, NL, NG)], &dpos);
                 atomic_max_nonneg(&fmax_sh[e], dpos);
             }
         }
@@ -512,16 +554,36 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
 constexpr int kLdsChunk = 256;
 constexpr int kLdsBuffers = 3;
 
-template <int NG, int NL, int WROWS, int NWAVES, int CH = kLdsChunk, int NBUF = kLdsBuffers>
+// MODE (see derived_power) > 0: the groups derived from the base group take
+// the base's term to their power instead of their own exponential; their
+// column factors (all NL of them, the reference's too) are relative to the
+// base's reference potentials, and so is their row shift.
+template <int NG, int NL, int WROWS, int NWAVES, int CH = kLdsChunk, int NBUF = kLdsBuffers, int MODE = 0>
 __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 8))) void batch_shared_lds_kernel(
     BatchArgs ba) {
     constexpr int E = NG * NL, L0 = NL / 2, THREADS = NWAVES * 64, RE = WROWS * E;
-    constexpr int NB = NL - 1;                        // factor columns per group
+    constexpr int NB = NL - 1;                        // factor columns of an own-exponential group
     constexpr int PER = (CH + THREADS - 1) / THREADS;  // staged columns per thread
+    constexpr int BASE = NG - 1;
+    static_assert(MODE == 0 || NG == 3, "shared exponentials across groups need three eps groups");
     static_assert(RE <= 64, "one lane per (row, combination) of a wave in the epilogue");
     static_assert(CH % 256 == 0 && (NBUF == 2 || NBUF == 3), "chunk of 64 lanes x 4 columns, 2 or 3 buffers");
-    __shared__ double h_lds[NBUF][NG][CH];
-    __shared__ float b_lds[NBUF][NG * NB][CH];
+    // staged slots: one potential row per own-exponential group, NB factor rows
+    // per such group and NL per derived group
+    constexpr auto pw = [](int g) constexpr { return derived_power(MODE, g, NG); };
+    constexpr auto hslot = [pw](int g) constexpr {
+        int n = 0;
+        for (int x = 0; x < g; ++x) n += pw(x) ? 0 : 1;
+        return n;
+    };
+    constexpr auto bslot = [pw](int g) constexpr {
+        int n = 0;
+        for (int x = 0; x < g; ++x) n += pw(x) ? NL : NL - 1;
+        return n;
+    };
+    constexpr int NH = hslot(NG), NBS = bslot(NG);
+    __shared__ double h_lds[NBUF][NH][CH];
+    __shared__ float b_lds[NBUF][NBS][CH];
     __shared__ int tile_sh;
     __shared__ int bad_row[NWAVES * WROWS];
     __shared__ float fmax_sh[E];
@@ -565,25 +627,31 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
             rowp[r] = A + (long long)row * pd.lda;
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
-                fk[r][g] = pot[(long long)(g * NL + L0) * pd.pot_ld + row] * kg[g];
+                const int gr = pw(g) ? BASE : g;  // whose reference potential shifts this group's rows
+                fk[r][g] = pot[(long long)(gr * NL + L0) * pd.pot_ld + row] * kg[g];
 #pragma unroll
                 for (int l = 0; l < NL; ++l) s[r][g * NL + l] = 0.f;
             }
         }
         // staging: thread t holds columns t + p * THREADS of a chunk (zero-mass
         // padding beyond ncols4); loaded into registers one chunk ahead
-        double hv[NG][PER];
-        float bv[NG * NB][PER];
+        double hv[NH][PER];
+        float bv[NBS][PER];
 #define WOTK_LDS_FETCH(k)                                                                        \
     _Pragma("unroll") for (int p = 0; p < PER; ++p) {                                           \
         const int j_ = tid + p * THREADS;                                                       \
         const int col_ = (k) * CH + j_;                                                         \
         const bool ok_ = j_ < CH && col_ < ncols4;                                              \
         _Pragma("unroll") for (int g = 0; g < NG; ++g) {                                        \
-            hv[g][p] = ok_ ? h[(long long)(g * NL + L0) * pd.other_ld + col_] : -INFINITY;      \
-            _Pragma("unroll") for (int i = 0; i < NB; ++i) {                                    \
-                const int l_ = i < L0 ? i : i + 1;                                              \
-                bv[g * NB + i][p] = ok_ ? B[(long long)(g * NL + l_) * pd.other_ld + col_] : 0.f; \
+            if (pw(g)) {                                                                        \
+                _Pragma("unroll") for (int l = 0; l < NL; ++l)                                  \
+                    bv[bslot(g) + l][p] = ok_ ? B[(long long)(g * NL + l) * pd.other_ld + col_] : 0.f; \
+            } else {                                                                            \
+                hv[hslot(g)][p] = ok_ ? h[(long long)(g * NL + L0) * pd.other_ld + col_] : -INFINITY; \
+                _Pragma("unroll") for (int i = 0; i < NB; ++i) {                                \
+                    const int l_ = i < L0 ? i : i + 1;                                          \
+                    bv[bslot(g) + i][p] = ok_ ? B[(long long)(g * NL + l_) * pd.other_ld + col_] : 0.f; \
+                }                                                                               \
             }                                                                                   \
         }                                                                                       \
     }
@@ -591,8 +659,8 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
     _Pragma("unroll") for (int p = 0; p < PER; ++p) {                                           \
         const int j_ = tid + p * THREADS;                                                       \
         if (j_ < CH) {                                                                          \
-            _Pragma("unroll") for (int g = 0; g < NG; ++g) h_lds[buf][g][j_] = hv[g][p];        \
-            _Pragma("unroll") for (int x = 0; x < NG * NB; ++x) b_lds[buf][x][j_] = bv[x][p];   \
+            _Pragma("unroll") for (int x = 0; x < NH; ++x) h_lds[buf][x][j_] = hv[x][p];        \
+            _Pragma("unroll") for (int x = 0; x < NBS; ++x) b_lds[buf][x][j_] = bv[x][p];       \
         }                                                                                       \
     }
         WOTK_LDS_FETCH(0)
@@ -607,9 +675,7 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
             if (k + 1 < nchunks) {  // in flight while this chunk computes
                 WOTK_LDS_FETCH(k + 1)
             }
-#pragma unroll
-            for (int sub = 0; sub < CH / 256; ++sub) {
-            const int cl = sub * 256 + lane * 4;  // column within the chunk
+            const int cl = lane * 4;  // column within the chunk
             const int c = k * CH + cl;
             float4 ccur[WROWS];
 #pragma unroll
@@ -627,31 +693,51 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
                     cv[r][2] = ccur[r].z;
                     cv[r][3] = ccur[r].w;
                 }
-
+                // the base group first: derived groups take powers of its terms
+                float tb[WROWS][4];
 #pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    const double2 h01 = *reinterpret_cast<const double2 *>(&h_lds[buf][g][cl]);
-                    const double2 h23 = *reinterpret_cast<const double2 *>(&h_lds[buf][g][cl + 2]);
-                    const double hk[4] = {h01.x * kg[g], h01.y * kg[g], h23.x * kg[g], h23.y * kg[g]};
-                    float4 bf[NB];
+                for (int gi = 0; gi < NG; ++gi) {
+                    const int g = NG - 1 - gi;
+                    if (pw(g)) {
+                        float4 bf[NL];
 #pragma unroll
-                    for (int i = 0; i < NB; ++i) bf[i] = *reinterpret_cast<const float4 *>(&b_lds[buf][g * NB + i][cl]);
+                        for (int l = 0; l < NL; ++l) bf[l] = *reinterpret_cast<const float4 *>(&b_lds[buf][bslot(g) + l][cl]);
 #pragma unroll
-                    for (int r = 0; r < WROWS; ++r) {
-                        float tq[4];
+                        for (int r = 0; r < WROWS; ++r) {
+                            float tq[4];
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) tq[q] = bexp2_split(fma(-kg[g], cv[r][q], hk[q] + fk[r][g]));
-                        s[r][g * NL + L0] += (tq[0] + tq[1]) + (tq[2] + tq[3]);
-                        int i = 0;
+                            for (int q = 0; q < 4; ++q) tq[q] = powi(tb[r][q], pw(g));
 #pragma unroll
-                        for (int l = 0; l < NL; ++l)
-                            if (l != L0) {
-                                s[r][g * NL + l] = fmaf(tq[3], bf[i].w, fmaf(tq[2], bf[i].z, fmaf(tq[1], bf[i].y, fmaf(tq[0], bf[i].x, s[r][g * NL + l]))));
-                                ++i;
+                            for (int l = 0; l < NL; ++l)
+                                s[r][g * NL + l] = fmaf(tq[3], bf[l].w, fmaf(tq[2], bf[l].z, fmaf(tq[1], bf[l].y, fmaf(tq[0], bf[l].x, s[r][g * NL + l]))));
+                        }
+                    } else {
+                        const double2 h01 = *reinterpret_cast<const double2 *>(&h_lds[buf][hslot(g)][cl]);
+                        const double2 h23 = *reinterpret_cast<const double2 *>(&h_lds[buf][hslot(g)][cl + 2]);
+                        const double hk[4] = {h01.x * kg[g], h01.y * kg[g], h23.x * kg[g], h23.y * kg[g]};
+                        float4 bf[NB];
+#pragma unroll
+                        for (int i = 0; i < NB; ++i) bf[i] = *reinterpret_cast<const float4 *>(&b_lds[buf][bslot(g) + i][cl]);
+#pragma unroll
+                        for (int r = 0; r < WROWS; ++r) {
+                            float tq[4];
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) tq[q] = bexp2_split(fma(-kg[g], cv[r][q], hk[q] + fk[r][g]));
+                            if (g == BASE) {
+#pragma unroll
+                                for (int q = 0; q < 4; ++q) tb[r][q] = tq[q];
                             }
+                            s[r][g * NL + L0] += (tq[0] + tq[1]) + (tq[2] + tq[3]);
+                            int i = 0;
+#pragma unroll
+                            for (int l = 0; l < NL; ++l)
+                                if (l != L0) {
+                                    s[r][g * NL + l] = fmaf(tq[3], bf[i].w, fmaf(tq[2], bf[i].z, fmaf(tq[1], bf[i].y, fmaf(tq[0], bf[i].x, s[r][g * NL + l]))));
+                                    ++i;
+                                }
+                        }
                     }
                 }
-            }
             }
             // with two buffers, every wave must be done with chunk k - 1 (same buffer
             // as k + 1) before it is overwritten; with three, the barrier that ended
@@ -684,7 +770,8 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
         __syncthreads();
         // combinations out of range go to the repair kernel (whole groups when the
         // reference failed); no barrier sits under a condition
-        const int m = lane < RE ? widen_to_groups<NG, NL>(bad_row[wave * WROWS + my_r]) : 0;
+        const int nmode = nxt[0].share;  // the reading sweep's mode: which reference each factor is relative to
+        const int m = lane < RE ? widen_to_groups<NG, NL>(bad_row[wave * WROWS + my_r], nmode) : 0;
         const bool row_ok = mine && !((m >> my_e) & 1);
         if (lane < RE) {
             const double p =
@@ -695,10 +782,10 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
         }
         __syncthreads();
         if (row_ok) {
-            const int ref = (my_e / NL) * NL + L0;
+            const int gref = (my_e / NL) * NL + L0;  // k of the combination's own group
             float dpos;
             ba.fac[pd.pot_off + (long long)my_e * pd.pot_ld + my_row] =
-                factor_of(nxt[ref].k, pot_sh[wave][lane], pot_sh[wave][my_r * E + ref], &dpos);
+                factor_of(nxt[gref].k, pot_sh[wave][lane], pot_sh[wave][my_r * E + factor_ref(nmode, my_e, NL, NG)], &dpos);
             atomic_max_nonneg(&fmax_sh[my_e], dpos);
         }
         __syncthreads();
@@ -748,9 +835,10 @@ __global__ __launch_bounds__(THREADS) void batch_repair_kernel(BatchArgs ba) {
         }
         __syncthreads();
         if (ba.group && tid < E && ((mask >> tid) & 1)) {
-            const int ref = (tid / ba.group) * ba.group + ba.group / 2;
+            const int ref = factor_ref(nxt[0].share, tid, ba.group, E / ba.group);
+            const int gref = (tid / ba.group) * ba.group + ba.group / 2;  // k of the combination's own group
             float dpos;
-            ba.fac[pd.pot_off + (long long)tid * pd.pot_ld + row] = factor_of(nxt[ref].k, pot_sh[tid], pot_sh[ref], &dpos);
+            ba.fac[pd.pot_off + (long long)tid * pd.pot_ld + row] = factor_of(nxt[gref].k, pot_sh[tid], pot_sh[ref], &dpos);
             atomic_max_nonneg(&ba.fmax[pd.pair * E + tid], dpos);
         }
         __syncthreads();
@@ -903,9 +991,15 @@ typedef int (*SweepFn)(const BatchArgs &, const BatchArgs &, int, int, int, int 
 constexpr int kRepairThreads = 256;
 typedef int (*OccFn)();  // resident workgroups per CU
 
+// compiled modes (powers of groups 1 and 0, see derived_power) of the
+// LDS-staged kernel for three eps groups of three combinations
+constexpr int kDerivedModes[] = {2 | (10 << 8), 2 | (4 << 8), 2};
+constexpr int kNumDerivedModes = sizeof(kDerivedModes) / sizeof(kDerivedModes[0]);
+
 struct BatchVariant {
     SweepFn sweep;
     OccFn occupancy;
+    SweepFn derived[kNumDerivedModes];  // per kDerivedModes entry; nullptr where not compiled
 };
 
 template <int E, int V>
@@ -932,7 +1026,7 @@ static int occupancy() {
 
 template <int E, int... V>
 static constexpr std::array<BatchVariant, sizeof...(V)> variants_of(std::integer_sequence<int, V...>) {
-    return {BatchVariant{launch_sweeps<E, V>, occupancy<E, V>}...};
+    return {BatchVariant{launch_sweeps<E, V>, occupancy<E, V>, {}}...};
 }
 
 template <int E>
@@ -957,22 +1051,24 @@ static const BatchVariant *pick(int E) {
 
 // shared-exponential launchers per (groups, group size, tile variant of kBatchSharedTiles)
 // the kernel of variant V: register-only (wave_rows == 0) or LDS-staged
-template <int NG, int NL, int V>
+template <int NG, int NL, int V, int MODE = 0>
 static constexpr void (*shared_kernel_of())(BatchArgs) {
     constexpr TileShape ts = kBatchSharedTiles[V];
-    if constexpr (ts.wave_rows == 0)
+    if constexpr (ts.wave_rows == 0) {
+        static_assert(MODE == 0, "derived groups need the LDS-staged kernel");
         return batch_shared_kernel<NG, NL, ts.rows, ts.threads>;
-    else
-        return batch_shared_lds_kernel<NG, NL, ts.wave_rows, ts.threads / 64>;
+    } else {
+        return batch_shared_lds_kernel<NG, NL, ts.wave_rows, ts.threads / 64, kLdsChunk, kLdsBuffers, MODE>;
+    }
 }
 
-template <int NG, int NL, int V>
+template <int NG, int NL, int V, int MODE = 0>
 static int launch_shared(const BatchArgs &af, const BatchArgs &ag, int grid_f, int grid_g, int grid_r, int *iter,
                          int *queues, hipStream_t s) {
     constexpr int T = kBatchSharedTiles[V].threads;
     static_assert(kBatchSharedTiles[V].wave_rows == 0 || kBatchSharedTiles[V].rows == kBatchSharedTiles[V].wave_rows * T / 64,
                   "LDS variants: rows = waves x rows per wave");
-    constexpr auto kern = shared_kernel_of<NG, NL, V>();
+    constexpr auto kern = shared_kernel_of<NG, NL, V, MODE>();
     kern<<<grid_f, T, 0, s>>>(af);
     batch_repair_kernel<NG * NL, kRepairThreads><<<grid_r, kRepairThreads, 0, s>>>(af);
     kern<<<grid_g, T, 0, s>>>(ag);
@@ -991,9 +1087,22 @@ static int occupancy_shared() {
     return per < 1 ? 1 : per;
 }
 
+template <int NG, int NL, int V, int M>
+static constexpr SweepFn derived_sweep() {
+    if constexpr (NG == 3 && NL == 3 && kBatchSharedTiles[V].wave_rows > 0)
+        return launch_shared<NG, NL, V, kDerivedModes[M]>;
+    else
+        return nullptr;
+}
+
+template <int NG, int NL, int V, int... M>
+static constexpr BatchVariant shared_variant(std::integer_sequence<int, M...>) {
+    return BatchVariant{launch_shared<NG, NL, V>, occupancy_shared<NG, NL, V>, {derived_sweep<NG, NL, V, M>()...}};
+}
+
 template <int NG, int NL, int... V>
 static constexpr std::array<BatchVariant, sizeof...(V)> shared_variants_of(std::integer_sequence<int, V...>) {
-    return {BatchVariant{launch_shared<NG, NL, V>, occupancy_shared<NG, NL, V>}...};
+    return {shared_variant<NG, NL, V>(std::make_integer_sequence<int, kNumDerivedModes>())...};
 }
 
 template <int NG, int NL>
@@ -1028,7 +1137,7 @@ static int persistent_grid(const BatchVariant &v) {
 // object it holds is released by its destructor, whatever path ends the run.
 struct wotk_batch_run {
     BatchArgs af{}, ag{};
-    SweepFn sweep = nullptr;
+    SweepFn sweep[1 + kNumDerivedModes] = {};  // 0: own exponentials per group; i: kDerivedModes[i - 1]
     int grid_f = 0, grid_g = 0, grid_r = 0;
     int E = 0, n_pairs = 0, group = 0, graph_iters = 0;
     int64_t n_iter = 0, done = 0;
@@ -1036,8 +1145,9 @@ struct wotk_batch_run {
     std::vector<PairDev> pf, pg;
     std::vector<int64_t> n_i, n_j;
     std::vector<double> eps_final;  // [pair][E]
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned char> mode_ix;  // sweep index per iteration (n_iter + 1)
+    hipGraph_t graph[1 + kNumDerivedModes] = {};
+    hipGraphExec_t exec[1 + kNumDerivedModes] = {};
     hipStream_t cs = nullptr;  // capture stream
     bool capturing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> timing;
@@ -1051,8 +1161,10 @@ struct wotk_batch_run {
             if (ev.first) (void)hipEventDestroy(ev.first);
             if (ev.second) (void)hipEventDestroy(ev.second);
         }
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        for (hipGraphExec_t x : exec)
+            if (x) (void)hipGraphExecDestroy(x);
+        for (hipGraph_t x : graph)
+            if (x) (void)hipGraphDestroy(x);
         if (cs) (void)hipStreamDestroy(cs);
     }
 };
@@ -1170,6 +1282,42 @@ extern "C" int wotk_batch_begin(const float *cost, const wotk_batch_pair *pairs,
             }
             run->eps_final[(size_t)p * E + e] = table[((size_t)n_iter * n_pairs + p) * E + e].eps;
         }
+    // exponentials shared across eps groups: per iteration, the compiled mode
+    // whose powers are k_g / k_base of every pair within 1e-10 relative (the
+    // exponent then differs from the group's own by < 1e-10 of itself)
+    run->mode_ix.assign((size_t)n_iter + 1, 0);
+    bool any_derived = false;
+    for (const SweepFn fn : variant->derived) any_derived = any_derived || fn;
+    if ((flags & WOTK_BATCH_SHARE_EPS) && group && E / group == 3 && any_derived) {
+        const int L0 = group / 2;
+        for (int64_t n = 0; n <= n_iter; ++n) {
+            int pw[2] = {0, 0};  // common integer power of groups 0 and 1 over every pair (0: none)
+            for (int g = 0; g < 2; ++g) {
+                int m = -1;
+                for (int p = 0; p < n_pairs && m != 0; ++p) {
+                    const IterParam *t = &table[((size_t)n * n_pairs + p) * E];
+                    const double r = t[g * group + L0].k / t[2 * group + L0].k;
+                    const long mr = std::lround(r);
+                    const int ok = (mr >= 2 && mr < 32 && std::fabs(r - (double)mr) <= 1e-10 * (double)mr) ? (int)mr : 0;
+                    m = (m < 0 || m == ok) ? ok : 0;
+                }
+                pw[g] = m > 0 ? m : 0;
+            }
+            // the compiled mode with these powers, else with group 0 on its own exponential
+            int ix = 0;
+            for (int a = 0; a < 2 && !ix; ++a) {
+                const int want = pw[1] | ((a == 0 ? pw[0] : 0) << 8);
+                for (int i = 0; i < kNumDerivedModes && want; ++i)
+                    if (kDerivedModes[i] == want && variant->derived[i]) {
+                        ix = i + 1;
+                        break;
+                    }
+            }
+            run->mode_ix[(size_t)n] = (unsigned char)ix;
+            const int code = ix ? kDerivedModes[ix - 1] : 0;
+            for (size_t q = 0; q < (size_t)n_pairs * E; ++q) table[(size_t)n * n_pairs * E + q].share = code;
+        }
+    }
     WOTK_HIP(hipMemcpyAsync(w.pairs_f, pf.data(), sizeof(PairDev) * n_pairs, hipMemcpyHostToDevice, s));
     WOTK_HIP(hipMemcpyAsync(w.pairs_g, pg.data(), sizeof(PairDev) * n_pairs, hipMemcpyHostToDevice, s));
     WOTK_HIP(hipMemcpyAsync(w.tiles_f, tf.data(), sizeof(Tile) * tf.size(), hipMemcpyHostToDevice, s));
@@ -1208,7 +1356,8 @@ extern "C" int wotk_batch_begin(const float *cost, const wotk_batch_pair *pairs,
     const int grid = persistent_grid(*variant);
     run->grid_f = (int)std::min<int64_t>(grid, n_tf);
     run->grid_g = (int)std::min<int64_t>(grid, n_tg);
-    run->sweep = variant->sweep;
+    run->sweep[0] = variant->sweep;
+    for (int i = 0; i < kNumDerivedModes; ++i) run->sweep[1 + i] = variant->derived[i];
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         ncu = 256;
@@ -1219,17 +1368,23 @@ extern "C" int wotk_batch_begin(const float *cost, const wotk_batch_pair *pairs,
     run->n_iter = n_iter;
     run->graph_iters = graph_iters;
     if (graph_iters > 0 && n_iter >= graph_iters) {
-        // capture graph_iters iterations once on a private stream, replay them
+        // capture graph_iters iterations of each sweep the schedule uses once on
+        // a private stream, replay them
         WOTK_HIP(hipStreamCreateWithFlags(&run->cs, hipStreamNonBlocking));
-        WOTK_HIP(hipStreamBeginCapture(run->cs, hipStreamCaptureModeThreadLocal));
-        run->capturing = true;
-        for (int t = 0; t < graph_iters; ++t) {
-            const int rc = run->sweep(run->af, run->ag, run->grid_f, run->grid_g, run->grid_r, w.iter, w.queues, run->cs);
-            if (rc) return rc;  // the destructor ends the capture and drops the partial graph
+        for (int ix = 0; ix <= kNumDerivedModes; ++ix) {
+            if (std::find(run->mode_ix.begin(), run->mode_ix.end() - 1, (unsigned char)ix) == run->mode_ix.end() - 1)
+                continue;
+            WOTK_HIP(hipStreamBeginCapture(run->cs, hipStreamCaptureModeThreadLocal));
+            run->capturing = true;
+            for (int t = 0; t < graph_iters; ++t) {
+                const int rc = run->sweep[ix](run->af, run->ag, run->grid_f, run->grid_g, run->grid_r, w.iter, w.queues,
+                                              run->cs);
+                if (rc) return rc;  // the destructor ends the capture and drops the partial graph
+            }
+            run->capturing = false;
+            WOTK_HIP(hipStreamEndCapture(run->cs, &run->graph[ix]));
+            WOTK_HIP(hipGraphInstantiate(&run->exec[ix], run->graph[ix], nullptr, nullptr, 0));
         }
-        run->capturing = false;
-        WOTK_HIP(hipStreamEndCapture(run->cs, &run->graph));
-        WOTK_HIP(hipGraphInstantiate(&run->exec, run->graph, nullptr, nullptr, 0));
     }
     *run_out = run.release();
     return WOTK_OK;
@@ -1247,15 +1402,23 @@ extern "C" int wotk_batch_advance(wotk_batch_run *run, int64_t n_iterations, int
         WOTK_HIP(hipEventCreate(&b));
         run->timing.back().second = b;
         WOTK_HIP(hipEventRecord(a, s));
-        while (run->exec && todo >= run->graph_iters) {
-            WOTK_HIP(hipGraphLaunch(run->exec, s));
-            run->done += run->graph_iters;
-            todo -= run->graph_iters;
-        }
-        for (; todo > 0; --todo, ++run->done) {
-            const int rc = run->sweep(run->af, run->ag, run->grid_f, run->grid_g, run->grid_r, run->w.iter,
-                                      run->w.queues, s);
-            if (rc) return rc;
+        while (todo > 0) {
+            // a graph replay where the next graph_iters iterations share one sweep
+            const int ix = run->mode_ix[(size_t)run->done];
+            int64_t len = 1;
+            const int64_t cap = std::min<int64_t>(todo, run->graph_iters);
+            while (len < cap && run->mode_ix[(size_t)(run->done + len)] == ix) ++len;
+            if (run->exec[ix] && len == run->graph_iters) {
+                WOTK_HIP(hipGraphLaunch(run->exec[ix], s));
+                run->done += run->graph_iters;
+                todo -= run->graph_iters;
+            } else {
+                const int rc = run->sweep[ix](run->af, run->ag, run->grid_f, run->grid_g, run->grid_r, run->w.iter,
+                                              run->w.queues, s);
+                if (rc) return rc;
+                ++run->done;
+                --todo;
+            }
         }
         WOTK_HIP(hipEventRecord(b, s));
     }
@@ -1303,6 +1466,16 @@ extern "C" int wotk_batch_potentials(wotk_batch_run *run, double *f_out, double 
         stats->epsilon_final = run->eps_final[0];
         stats->stabilisations = (int64_t)exact_rows;
         stats->gap_checks = run->group;
+    }
+    return WOTK_OK;
+}
+
+extern "C" int wotk_batch_modes(const wotk_batch_run *run, int32_t *modes, int64_t n) {
+    WOTK_REQUIRE(run != nullptr, "no batch run");
+    WOTK_REQUIRE(modes != nullptr && n >= run->n_iter, "modes needs %lld entries", (long long)run->n_iter);
+    for (int64_t i = 0; i < run->n_iter; ++i) {
+        const int ix = run->mode_ix[(size_t)i];
+        modes[i] = ix ? kDerivedModes[ix - 1] : 0;
     }
     return WOTK_OK;
 }

@@ -137,18 +137,21 @@ def prepare_time_course(coords, growth, pairs=None, cost_mode='f64', device='cud
 
 def solve_time_course(coords, growth, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_iter=3000,
                       tau=1e4, epsilon0=1.0, extra_iter=1000, inner_iter_max=50, graph_iters=50,
-                      cost_mode='f64', pairs=None, group_size=None, share_exp=True, device='cuda'):
+                      cost_mode='f64', pairs=None, group_size=None, share_exp=True, share_eps=True,
+                      device='cuda'):
     """Solve every consecutive day pair (or the given index pairs) for every
     (epsilon, lambda1) of the grid epsilons x lambda1s.  ``coords[t]`` are the
     PCA coordinates of day t, ``growth[t]`` its cells' growth (G = growth, dt=1).
     ``group_size`` splits the grid into batched runs of that many combinations
     (each cost entry is then read once per group instead of once overall).
     ``share_exp`` lets combinations of equal epsilon share one exponential per
-    cost entry (libwotk ``WOTK_BATCH_SHARE_EXP``)."""
+    cost entry (libwotk ``WOTK_BATCH_SHARE_EXP``); ``share_eps`` also lets eps
+    groups in integer ratios of k = log2(e)/eps take powers of the largest
+    eps's exponential once the schedule has converged (``WOTK_BATCH_SHARE_EPS``)."""
     tc = prepare_time_course(coords, growth, pairs=pairs, cost_mode=cost_mode, device=device)
     return solve_prepared(tc, epsilons, lambda1s, lambda2=lambda2, scaling_iter=scaling_iter, tau=tau,
                           epsilon0=epsilon0, extra_iter=extra_iter, inner_iter_max=inner_iter_max,
-                          graph_iters=graph_iters, group_size=group_size, share_exp=share_exp)
+                          graph_iters=graph_iters, group_size=group_size, share_exp=share_exp, share_eps=share_eps)
 
 
 class TimeCourseRun:
@@ -160,11 +163,12 @@ class TimeCourseRun:
     tensor, one growth vector per pair at ``bp.g_offset`` (``growth_per_combo``
     False) or one per (pair, combination) ([combination][I_p] from
     ``bp.g_offset``).  ``advance(n)`` runs the next n iterations of the
-    schedule; ``result()`` returns the potentials as a :class:`BatchResult`."""
+    schedule; ``result()`` returns the potentials as a :class:`BatchResult`;
+    ``modes()`` which iterations took powers of one exponential (``share_eps``)."""
 
     def __init__(self, tc, epsilon, lambda1, lambda2, scaling_iter=3000, epsilon0=1.0, extra_iter=1000,
                  inner_iter_max=50, graph_iters=50, share_exp=True, G=None, growth_per_combo=False,
-                 combos=None, bps=None):
+                 combos=None, bps=None, share_eps=True):
         torch = _dev.torch_mod()
         from .. import tune
         tune.apply_persisted()
@@ -187,10 +191,11 @@ class TimeCourseRun:
         self.G = tc.G if G is None else G
         h = ctypes.c_void_p()
         P = ctypes.POINTER(ctypes.c_double)
+        flags = (_native.BATCH_SHARE_EXP if share_exp else 0) | (_native.BATCH_SHARE_EPS if share_exp and share_eps else 0)
         _native.check(L.wotk_batch_begin(_dev._ptr(tc.buf), arr, n_pairs, _dev._ptr(self.G), int(bool(growth_per_combo)),
                                          self.E, eps.ctypes.data_as(P), l1.ctypes.data_as(P), l2.ctypes.data_as(P),
                                          int(scaling_iter), float(epsilon0), int(extra_iter), int(inner_iter_max),
-                                         int(graph_iters), _native.BATCH_SHARE_EXP if share_exp else 0,
+                                         int(graph_iters), flags,
                                          _dev._ptr(self.ws), self.ws.numel(), ctypes.byref(h), _dev._stream(torch)))
         self._h = h
         self.remaining = self.n_iter
@@ -230,6 +235,16 @@ class TimeCourseRun:
         eps_pe = np.array(epsf[:], dtype=np.float64).reshape(tc.n_pairs, E)
         return BatchResult(tc.buf, self._bps, self.combos, fs, gs, d, shapes, eps_pe)
 
+    def modes(self):
+        """Per iteration: the powers eps groups 1 (bits 0-7) and 0 (bits 8-15)
+        take of the last group's exponential (``share_eps``), 0 where every
+        group uses its own."""
+        if self._h is None:
+            raise RuntimeError('batch run already closed')
+        out = (ctypes.c_int32 * max(1, self.n_iter))()
+        _native.check(_native.lib().wotk_batch_modes(self._h, out, self.n_iter))
+        return np.array(out[:self.n_iter], dtype=np.int32)
+
     def close(self):
         if self._h is not None:
             _native.lib().wotk_batch_end(self._h)
@@ -259,7 +274,8 @@ def grid_arrays(n_pairs, epsilons, lambda1s, lambda2):
 
 
 def solve_prepared(tc, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_iter=3000, tau=1e4, epsilon0=1.0,
-                   extra_iter=1000, inner_iter_max=50, graph_iters=50, group_size=None, share_exp=True):
+                   extra_iter=1000, inner_iter_max=50, graph_iters=50, group_size=None, share_exp=True,
+                   share_eps=True):
     """``solve_time_course`` on costs already built by ``prepare_time_course``."""
     torch = _dev.torch_mod()
     combos, eps, l1, l2 = grid_arrays(tc.n_pairs, epsilons, lambda1s, lambda2)
@@ -274,7 +290,7 @@ def solve_prepared(tc, epsilons=(0.05,), lambda1s=(1.0,), lambda2=50.0, scaling_
     for grp in groups:
         with TimeCourseRun(tc, eps[:, grp], l1[:, grp], l2[:, grp], scaling_iter=scaling_iter, epsilon0=epsilon0,
                            extra_iter=extra_iter, inner_iter_max=inner_iter_max, graph_iters=graph_iters,
-                           share_exp=share_exp, combos=[combos[e] for e in grp]) as run:
+                           share_exp=share_exp, share_eps=share_eps, combos=[combos[e] for e in grp]) as run:
             run.advance(run.n_iter)
             r = run.result()
         for k in range(tc.n_pairs):
