@@ -176,12 +176,15 @@ int wotk_plan(const float *C, int64_t ldc, int64_t I, int64_t J, const double *f
  * are recomputed exactly.  Ignored (the per-combination kernel runs) when the
  * grid has no such grouping.
  * flags & WOTK_BATCH_SHARE_EPS (with WOTK_BATCH_SHARE_EXP, three groups of
- * three): at every iteration where k = log2(e)/eps of groups 0 and 1 is an
- * integer multiple m of the last group's (within 1e-10 relative, for every
- * pair) -- e.g. eps {0.01, 0.05, 0.1} once the eps schedule has converged --
- * those groups take the last group's exponential to the power m (FP32
- * products) instead of their own; compiled powers (group 1, group 0): (2, 10),
- * (2, 4), (2, own).  wotk_batch_modes reports the powers per iteration.
+ * three): groups 0 and 1 reuse the last group's FP64 exponent.  At every
+ * iteration where k = log2(e)/eps of a group is an integer multiple m >= 1 of
+ * the last group's (within 1e-10 relative, for every pair) -- e.g. eps
+ * {0.01, 0.05, 0.1} once the eps schedule has converged -- it takes the last
+ * group's exponential to the power m (FP32 products) instead of its own;
+ * otherwise its exponent is r = k_g / k_last times the last group's (ratios up
+ * to 2^16), with its own exponential.  Compiled (group 1, group 0): (2, 10),
+ * (2, 4), (2, r), (1, 1) and (r, r).  wotk_batch_modes reports the mode per
+ * iteration.
  * stats: sweeps / sweep_ms as for the single solver; stabilisations = rows
  * recomputed exactly (fixed shift or factors out of range); gap_checks =
  * combinations per shared-exponential group (0: one exponential per
@@ -237,8 +240,9 @@ int wotk_batch_potentials(wotk_batch_run *run, double *f_out, double *g_out, dou
                           wotk_stats *stats, void *stream);
 /* Per iteration of the run's schedule (n >= its iteration count, HOST
  * array): the powers groups 1 (bits 0-7) and 0 (bits 8-15) take of the last
- * group's exponential under WOTK_BATCH_SHARE_EPS, 0 where each group uses its
- * own.  No counterpart in the reference (it solves each combination alone). */
+ * group's exponential under WOTK_BATCH_SHARE_EPS (255: the real ratio of
+ * their exponents), 0 where each group uses its own exponent.  No counterpart
+ * in the reference (it solves each combination alone). */
 int wotk_batch_modes(const wotk_batch_run *run, int32_t *modes, int64_t n);
 void wotk_batch_end(wotk_batch_run *run);
 

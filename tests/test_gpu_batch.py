@@ -137,21 +137,36 @@ def test_batch_single_combo_equals_single_solver(batch):
     assert np.linalg.norm(R1 - R2) / np.linalg.norm(R1) < 1e-6
 
 
+REAL = 255  # a group whose k ratio to the base group's is not an integer
+COMPILED = (2 | (10 << 8), 2 | (4 << 8), 2 | (REAL << 8), 1 | (1 << 8), REAL | (REAL << 8))
+
+
+def _expected_mode(ratios):
+    """The library's choice for ratios k_g / k_base of groups 0 and 1: integer
+    powers within 1e-10, else the real ratio; the first compiled of (p1, p0),
+    (p1, real), (real, real)."""
+    pw = [int(round(r)) if abs(r - round(r)) <= 1e-10 * round(r) and round(r) >= 1 else REAL for r in ratios]
+    for want in (pw[1] | (pw[0] << 8), pw[1] | (REAL << 8), REAL | (REAL << 8)):
+        if want in COMPILED:
+            return want
+    raise AssertionError('no compiled mode')
+
+
 # eps grids whose k = log2(e)/eps stand in integer ratios to the largest eps's:
 # (groups 1, 0) take its exponential squared and to the 10th / 4th power, or
-# group 0 keeps its own (ratio 10/3)
+# group 0 takes the real ratio 10/3 of its exponent
 @pytest.mark.parametrize('epsilons,mode', [
     ((0.01, 0.05, 0.1), 2 | (10 << 8)),
     ((0.025, 0.05, 0.1), 2 | (4 << 8)),
-    ((0.03, 0.05, 0.1), 2),
+    ((0.03, 0.05, 0.1), 2 | (REAL << 8)),
 ])
 def test_shared_eps_powers_match_oracle(batch, epsilons, mode):
     """With WOTK_BATCH_SHARE_EPS, once the eps schedule has converged (here
-    from iteration 140 of 400: eps changes every 5 iterations, and
-    |k_g / k_base - m| <= 1e-10 m from e^-28 on) the smaller-eps groups take
-    powers of the largest eps's exponential; every plan still meets the
-    oracle gate, and differs from the run with each group's own exponential
-    by far less than the gate."""
+    from iteration 130 of 400: eps changes every 5 iterations, and
+    |k_g / k_base - m| <= 1e-10 m from e^-26..28 on) the smaller-eps groups
+    take powers of the largest eps's exponential, before that its exponent
+    times the real ratio; every plan still meets the oracle gate, and differs
+    from the run with each group's own exponent by far less than the gate."""
     sched = dict(scaling_iter=300, extra_iter=100, inner_iter_max=5, tau=1e4)
     coords, growth = _course(11, [150, 320, 200])
     lams = (0.5, 1.0, 5.0)
@@ -162,20 +177,17 @@ def test_shared_eps_powers_match_oracle(batch, epsilons, mode):
         run.advance(run.n_iter)
         modes = run.modes()
         res = run.result()
-    # the host's choice per iteration: the powers whose ratios hold within 1e-10,
-    # among the compiled modes (group 0 falls back to its own exponential)
+    # the host's choice per iteration
     e0 = np.array(epsilons)
-    compiled = (2 | (10 << 8), 2 | (4 << 8), 2)
     want = np.zeros(run.n_iter, dtype=np.int32)
     for n in range(run.n_iter):
         cur = (1.0 - e0) * np.exp(-float(min(n, 300) // 5)) + e0
-        r = cur[2] / cur[:2]
-        pw = [int(round(r[g])) if abs(r[g] - round(r[g])) <= 1e-10 * round(r[g]) and round(r[g]) >= 2 else 0
-              for g in (0, 1)]
-        full, part = pw[1] | (pw[0] << 8), pw[1]
-        want[n] = full if full in compiled else (part if part in compiled else 0)
+        want[n] = _expected_mode(cur[2] / cur[:2])
     assert np.array_equal(modes, want), (np.unique(modes), np.unique(want))
-    first = int(np.argmax(want != 0))
+    # eps0 for every group in the schedule's first step: one exponential for all
+    # nine; then real ratios until they converge to the grid's
+    assert (want[:5] == 1 | (1 << 8)).all() and (want[5:100] == REAL | (REAL << 8)).all()
+    first = 5 + int(np.argmax(want[5:] != REAL | (REAL << 8)))
     assert 100 < first < 200 and (want[200:] == mode).all()
     worst = 0.0
     for p in range(2):
@@ -195,9 +207,11 @@ def test_shared_eps_powers_match_oracle(batch, epsilons, mode):
     assert diff <= 2e-6
 
 
-def test_shared_eps_needs_the_flag_and_converged_ratios(batch):
-    """Without the flag, or while the schedule's eps ratios are not integers,
-    every group keeps its own exponential."""
+def test_shared_eps_modes_follow_the_schedule(batch):
+    """Without the flag every group keeps its own exponent; with it every
+    iteration shares the base group's: one exponential while all groups are at
+    eps0 (the schedule's first step), the real ratio while the ratios are not
+    integers."""
     coords, growth = _course(12, [90, 130])
     combos, eps, l1, l2 = batch.grid_arrays(1, (0.01, 0.05, 0.1), (0.5, 1.0, 5.0), 50.0)
     tc = batch.prepare_time_course(coords, growth)
@@ -205,4 +219,8 @@ def test_shared_eps_needs_the_flag_and_converged_ratios(batch):
                              (True, dict(scaling_iter=300, extra_iter=100, inner_iter_max=50))):
         with batch.TimeCourseRun(tc, eps, l1, l2, epsilon0=1.0, graph_iters=50, share_eps=share_eps,
                                  combos=combos, **sched) as run:
-            assert (run.modes() == 0).all(), share_eps
+            m = run.modes()
+            if share_eps:
+                assert (m[:50] == 1 | (1 << 8)).all() and (m[50:] == REAL | (REAL << 8)).all()
+            else:
+                assert (m == 0).all()

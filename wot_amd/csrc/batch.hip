@@ -56,8 +56,12 @@ struct IterParam {  // one (iteration, pair, combination)
 // schedule's eps has converged, e.g. eps {0.01, 0.05, 0.1}: k_0 = 10 k_2,
 // k_1 = 2 k_2 -- 2^(k_g y) = (2^(k_2 y))^m: a derived group g takes the
 // base's term to the power m instead of its own exponential, with column
-// factors relative to the base's reference potentials.  `mode` packs the
-// powers: group 1's in bits 0-7, group 0's in bits 8-15 (0: own exponential).
+// factors relative to the base's reference potentials.  For any other ratio
+// r = k_g / k_base the group still reuses the base's FP64 exponent: its own is
+// r times it (one FP64 product instead of the sum, FMA and clamp), with its
+// own exponential.  `mode` packs the powers: group 1's in bits 0-7, group 0's
+// in bits 8-15 (0: own exponent, kRealPower: r times the base's).
+constexpr int kRealPower = 255;
 __device__ __host__ __forceinline__ constexpr int derived_power(int mode, int g, int ng) {
     return ng != 3 ? 0 : g == 1 ? (mode & 0xff) : g == 0 ? ((mode >> 8) & 0xff) : 0;
 }
@@ -174,6 +178,13 @@ __device__ __forceinline__ const IterParam *params_at(const BatchArgs &ba, int i
 // which the row checks catch.
 __device__ __forceinline__ float bexp2_split(double y) {
     y = fmax(y, -16384.0);
+    const double n = rint(y);
+    return ldexpf(bexp2((float)(y - n)), (int)n);
+}
+
+// the same for an exponent already clamped (or a product of a clamped one with a
+// ratio up to 2^16: rint stays within int range)
+__device__ __forceinline__ float bexp2_split_nc(double y) {
     const double n = rint(y);
     return ldexpf(bexp2((float)(y - n)), (int)n);
 }
@@ -609,9 +620,11 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
         prev_pair = pd.pair;
         const IterParam *prm = params_at(ba, it, pd.pair);
         const IterParam *nxt = params_at(ba, it + ba.k_next, pd.pair);
-        double kg[NG];
+        double kg[NG], rg[NG];
 #pragma unroll
         for (int g = 0; g < NG; ++g) kg[g] = prm[g * NL + L0].k;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) rg[g] = pw(g) == kRealPower ? kg[g] / kg[BASE] : 1.0;
         const float *A = ba.cost + pd.a_off;
         const double *h = ba.other + pd.other_off;
         const float *B = ba.fac_other + pd.other_off;
@@ -693,8 +706,11 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
                     cv[r][2] = ccur[r].z;
                     cv[r][3] = ccur[r].w;
                 }
-                // the base group first: derived groups take powers of its terms
+                // the base group first: derived groups take powers of its terms, or
+                // multiples of its exponents
+                constexpr bool REAL = pw(0) == kRealPower || pw(1) == kRealPower;
                 float tb[WROWS][4];
+                double yb[WROWS][REAL ? 4 : 1];
 #pragma unroll
                 for (int gi = 0; gi < NG; ++gi) {
                     const int g = NG - 1 - gi;
@@ -705,8 +721,13 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
 #pragma unroll
                         for (int r = 0; r < WROWS; ++r) {
                             float tq[4];
+                            if (pw(g) == kRealPower) {
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) tq[q] = powi(tb[r][q], pw(g));
+                                for (int q = 0; q < 4; ++q) tq[q] = bexp2_split_nc(yb[r][REAL ? q : 0] * rg[g]);
+                            } else {
+#pragma unroll
+                                for (int q = 0; q < 4; ++q) tq[q] = powi(tb[r][q], pw(g));
+                            }
 #pragma unroll
                             for (int l = 0; l < NL; ++l)
                                 s[r][g * NL + l] = fmaf(tq[3], bf[l].w, fmaf(tq[2], bf[l].z, fmaf(tq[1], bf[l].y, fmaf(tq[0], bf[l].x, s[r][g * NL + l]))));
@@ -721,8 +742,16 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
 #pragma unroll
                         for (int r = 0; r < WROWS; ++r) {
                             float tq[4];
+                            if (REAL && g == BASE) {
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) tq[q] = bexp2_split(fma(-kg[g], cv[r][q], hk[q] + fk[r][g]));
+                                for (int q = 0; q < 4; ++q) {
+                                    yb[r][REAL ? q : 0] = fmax(fma(-kg[g], cv[r][q], hk[q] + fk[r][g]), -16384.0);
+                                    tq[q] = bexp2_split_nc(yb[r][REAL ? q : 0]);
+                                }
+                            } else {
+#pragma unroll
+                                for (int q = 0; q < 4; ++q) tq[q] = bexp2_split(fma(-kg[g], cv[r][q], hk[q] + fk[r][g]));
+                            }
                             if (g == BASE) {
 #pragma unroll
                                 for (int q = 0; q < 4; ++q) tb[r][q] = tq[q];
@@ -993,7 +1022,8 @@ typedef int (*OccFn)();  // resident workgroups per CU
 
 // compiled modes (powers of groups 1 and 0, see derived_power) of the
 // LDS-staged kernel for three eps groups of three combinations
-constexpr int kDerivedModes[] = {2 | (10 << 8), 2 | (4 << 8), 2};
+constexpr int kDerivedModes[] = {2 | (10 << 8), 2 | (4 << 8), 2 | (kRealPower << 8), 1 | (1 << 8),
+                                  kRealPower | (kRealPower << 8)};
 constexpr int kNumDerivedModes = sizeof(kDerivedModes) / sizeof(kDerivedModes[0]);
 
 struct BatchVariant {
@@ -1292,27 +1322,31 @@ extern "C" int wotk_batch_begin(const float *cost, const wotk_batch_pair *pairs,
         const int L0 = group / 2;
         for (int64_t n = 0; n <= n_iter; ++n) {
             int pw[2] = {0, 0};  // common integer power of groups 0 and 1 over every pair (0: none)
+            bool in_range = true;  // real ratios below 2^16 keep r * y within int range (y >= -16384)
             for (int g = 0; g < 2; ++g) {
                 int m = -1;
-                for (int p = 0; p < n_pairs && m != 0; ++p) {
+                for (int p = 0; p < n_pairs; ++p) {
                     const IterParam *t = &table[((size_t)n * n_pairs + p) * E];
                     const double r = t[g * group + L0].k / t[2 * group + L0].k;
+                    in_range = in_range && r > 0.0 && r <= 65536.0;
                     const long mr = std::lround(r);
-                    const int ok = (mr >= 2 && mr < 32 && std::fabs(r - (double)mr) <= 1e-10 * (double)mr) ? (int)mr : 0;
+                    const int ok = (mr >= 1 && mr < 32 && std::fabs(r - (double)mr) <= 1e-10 * (double)mr) ? (int)mr : 0;
                     m = (m < 0 || m == ok) ? ok : 0;
                 }
                 pw[g] = m > 0 ? m : 0;
             }
-            // the compiled mode with these powers, else with group 0 on its own exponential
+            // the compiled mode with these powers (a group without one takes the real
+            // ratio), else with group 0, then both, on the real ratio (power 1: equal
+            // eps, e.g. every group at eps0 before the schedule's first step)
+            const int p1 = pw[1] ? pw[1] : kRealPower, p0 = pw[0] ? pw[0] : kRealPower;
+            const int wants[3] = {p1 | (p0 << 8), p1 | (kRealPower << 8), kRealPower | (kRealPower << 8)};
             int ix = 0;
-            for (int a = 0; a < 2 && !ix; ++a) {
-                const int want = pw[1] | ((a == 0 ? pw[0] : 0) << 8);
-                for (int i = 0; i < kNumDerivedModes && want; ++i)
-                    if (kDerivedModes[i] == want && variant->derived[i]) {
+            for (int a = 0; a < 3 && !ix && in_range; ++a)
+                for (int i = 0; i < kNumDerivedModes; ++i)
+                    if (kDerivedModes[i] == wants[a] && variant->derived[i]) {
                         ix = i + 1;
                         break;
                     }
-            }
             run->mode_ix[(size_t)n] = (unsigned char)ix;
             const int code = ix ? kDerivedModes[ix - 1] : 0;
             for (size_t q = 0; q < (size_t)n_pairs * E; ++q) table[(size_t)n * n_pairs * E + q].share = code;

@@ -145,9 +145,11 @@ def solve_time_course(coords, growth, epsilons=(0.05,), lambda1s=(1.0,), lambda2
     ``group_size`` splits the grid into batched runs of that many combinations
     (each cost entry is then read once per group instead of once overall).
     ``share_exp`` lets combinations of equal epsilon share one exponential per
-    cost entry (libwotk ``WOTK_BATCH_SHARE_EXP``); ``share_eps`` also lets eps
-    groups in integer ratios of k = log2(e)/eps take powers of the largest
-    eps's exponential once the schedule has converged (``WOTK_BATCH_SHARE_EPS``)."""
+    cost entry (libwotk ``WOTK_BATCH_SHARE_EXP``); ``share_eps`` also lets
+    three eps groups share the last group's FP64 exponent: powers of its
+    exponential where k = log2(e)/eps stands in an integer ratio (the
+    converged schedule), the real multiple of its exponent elsewhere
+    (``WOTK_BATCH_SHARE_EPS``)."""
     tc = prepare_time_course(coords, growth, pairs=pairs, cost_mode=cost_mode, device=device)
     return solve_prepared(tc, epsilons, lambda1s, lambda2=lambda2, scaling_iter=scaling_iter, tau=tau,
                           epsilon0=epsilon0, extra_iter=extra_iter, inner_iter_max=inner_iter_max,
@@ -237,8 +239,8 @@ class TimeCourseRun:
 
     def modes(self):
         """Per iteration: the powers eps groups 1 (bits 0-7) and 0 (bits 8-15)
-        take of the last group's exponential (``share_eps``), 0 where every
-        group uses its own."""
+        take of the last group's exponential (``share_eps``; 255: the real
+        ratio of their exponents), 0 where every group uses its own exponent."""
         if self._h is None:
             raise RuntimeError('batch run already closed')
         out = (ctypes.c_int32 * max(1, self.n_iter))()
