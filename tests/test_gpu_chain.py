@@ -59,5 +59,9 @@ def test_glue_reorders_intermediate_cells(tm):
     t1 = AnnData(x1[perm], pd.DataFrame(index=[['p', 'q', 'r', 's'][i] for i in perm]),
                  pd.DataFrame(index=['u', 'v']))
     out = tm.glue_transport_maps(t0, t1)
-    np.testing.assert_allclose(out.X, x0 @ x1, rtol=1e-6)
+    np.testing.assert_allclose(out.X, x0 @ x1, rtol=1e-14)  # FP64, as the reference's np.dot
     assert list(out.obs.index) == ['a', 'b', 'c'] and list(out.var.index) == ['u', 'v']
+    out32 = tm.glue_transport_maps(t0, t1, precision='float32')
+    np.testing.assert_allclose(out32.X, x0 @ x1, rtol=1e-6)
+    with pytest.raises(ValueError):
+        tm.glue_transport_maps(t0, t1, precision='float16')

@@ -119,6 +119,9 @@ struct BatchArgs {
     const float *fmax_other;   // the same for the other side's factors (read)
     float *fmax_reset;         // cleared by this side's repair launch (the array this sweep consumed)
     int n_fmax;                // entries of each fmax array (pairs x combinations)
+    float *psi, *msh;          // LDS kernel: this side's column shifts (fraction, integer) per group, at
+                               // the group's reference slot of [E][pot_ld] (for the other side's next sweep)
+    const float *psi_other, *m_other;  // the same of the other side (read)
 };
 
 // queue combinations `mask` (bit e: combination e) of row `row` of side-pair
@@ -182,11 +185,38 @@ __device__ __forceinline__ float bexp2_split(double y) {
     return ldexpf(bexp2((float)(y - n)), (int)n);
 }
 
-// the same for an exponent already clamped (or a product of a clamped one with a
-// ratio up to 2^16: rint stays within int range)
+// the same for an exponent already clamped below (or a product of a clamped one
+// with a ratio up to 2^16: rint stays within int range)
 __device__ __forceinline__ float bexp2_split_nc(double y) {
     const double n = rint(y);
     return ldexpf(bexp2((float)(y - n)), (int)n);
+}
+
+// Exponent of a term split in FP32 arithmetic only (the LDS-staged kernel):
+//   y = H_j + F_i - k c,  H_j = M_j + psi_j and F_i = N_i + phi_i (integer M, N;
+//   |psi|, |phi| <= 1/2), k = k1 + k2 (FP32 high and low parts).
+//   P = fl(k1 c), n = rint(P): t = n - k1 c comes exactly from one FMA (|t| <=
+//   1/2 + ulp(P) / 2, rounded once), then t - k2 c + psi_j, and
+//   2^(y - phi_i) = ldexp(exp2(t - k2 c + psi_j), M_j + N_i - n).
+// The exponent's absolute error stays below 2^-23 whatever |y| is (the FP32
+// cost itself carries |k c| 2^-24); 2^phi_i is a per-row constant the
+// epilogue applies to the row sum.  mn = M_j + N_i: integers below 2^23 (the
+// shifts are clamped at +-2^22, where 2^y is 0 or inf in FP32 anyway).
+__device__ __forceinline__ float split_exp_f32(float c, float k1, float k2, float psi, float mn) {
+    const float n = rintf(k1 * c);
+    const float t = fmaf(-k2, c, fmaf(-k1, c, n));
+    return ldexpf(bexp2(t + psi), (int)(mn - n));
+}
+
+constexpr double kShiftClamp = 4194304.0;  // 2^22
+
+// fraction and integer part (as FP32) of a shift H (a potential times k),
+// clamped at +-2^22 (a NaN shift clamps low: its terms vanish)
+__device__ __forceinline__ void split_shift(double H, float *psi, float *m) {
+    H = fmin(fmax(H, -kShiftClamp), kShiftClamp);
+    const double M = rint(H);
+    *psi = (float)(H - M);
+    *m = (float)M;
 }
 
 __device__ __forceinline__ float4 bload4(const float *p) {
@@ -565,35 +595,47 @@ __global__ __launch_bounds__(THREADS) void batch_shared_kernel(BatchArgs ba) {
 constexpr int kLdsChunk = 256;
 constexpr int kLdsBuffers = 3;
 
-// MODE (see derived_power) > 0: the groups derived from the base group take
-// the base's term to their power instead of their own exponential; their
-// column factors (all NL of them, the reference's too) are relative to the
-// base's reference potentials, and so is their row shift.
+// MODE (see derived_power): a group with an integer power takes the base
+// group's term to that power; a group with a real ratio r takes r times the
+// base's exponent, formed in FP64 (bexp2_split) in such modes; every other
+// group forms its own exponent in FP32 (split_exp_f32) from the column shifts
+// the other side's epilogue wrote for it.  Groups other than those on their
+// own reference (power 0) have column factors for all NL combinations,
+// relative to the base's reference potentials, and their row shift is the
+// base's reference potential.
 template <int NG, int NL, int WROWS, int NWAVES, int CH = kLdsChunk, int NBUF = kLdsBuffers, int MODE = 0>
 __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 8))) void batch_shared_lds_kernel(
     BatchArgs ba) {
     constexpr int E = NG * NL, L0 = NL / 2, THREADS = NWAVES * 64, RE = WROWS * E;
-    constexpr int NB = NL - 1;                        // factor columns of an own-exponential group
     constexpr int PER = (CH + THREADS - 1) / THREADS;  // staged columns per thread
     constexpr int BASE = NG - 1;
     static_assert(MODE == 0 || NG == 3, "shared exponentials across groups need three eps groups");
     static_assert(RE <= 64, "one lane per (row, combination) of a wave in the epilogue");
     static_assert(CH % 256 == 0 && (NBUF == 2 || NBUF == 3), "chunk of 64 lanes x 4 columns, 2 or 3 buffers");
-    // staged slots: one potential row per own-exponential group, NB factor rows
-    // per such group and NL per derived group
     constexpr auto pw = [](int g) constexpr { return derived_power(MODE, g, NG); };
-    constexpr auto hslot = [pw](int g) constexpr {
+    // REAL: some group takes a real ratio of the base's FP64 exponent
+    constexpr bool REAL = pw(0) == kRealPower || pw(1) == kRealPower;
+    // how each group's terms are formed: FP32 split of its own exponent, the
+    // base's FP64 exponent (REAL modes), r times it, or an integer power
+    constexpr auto split32 = [pw](int g) constexpr { return pw(g) == 0 && !(REAL && g == NG - 1); };
+    constexpr auto full64 = [](int g) constexpr { return REAL && g == NG - 1; };
+    // staged slots: the column shift (fraction, integer) of every group whose
+    // exponent is formed here; NL - 1 factor rows of a group on its own
+    // reference, NL otherwise
+    constexpr auto staged = [split32, full64](int g) constexpr { return split32(g) || full64(g); };
+    constexpr auto xslot = [staged](int g) constexpr {
         int n = 0;
-        for (int x = 0; x < g; ++x) n += pw(x) ? 0 : 1;
+        for (int x = 0; x < g; ++x) n += staged(x) ? 1 : 0;
         return n;
     };
     constexpr auto bslot = [pw](int g) constexpr {
         int n = 0;
-        for (int x = 0; x < g; ++x) n += pw(x) ? NL : NL - 1;
+        for (int x = 0; x < g; ++x) n += pw(x) == 0 ? NL - 1 : NL;
         return n;
     };
-    constexpr int NH = hslot(NG), NBS = bslot(NG);
-    __shared__ double h_lds[NBUF][NH][CH];
+    constexpr int NX = xslot(NG), NBS = bslot(NG);
+    __shared__ float psi_lds[NBUF][NX][CH];
+    __shared__ float m_lds[NBUF][NX][CH];
     __shared__ float b_lds[NBUF][NBS][CH];
     __shared__ int tile_sh;
     __shared__ int bad_row[NWAVES * WROWS];
@@ -621,34 +663,47 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
         const IterParam *prm = params_at(ba, it, pd.pair);
         const IterParam *nxt = params_at(ba, it + ba.k_next, pd.pair);
         double kg[NG], rg[NG];
+        float k1[NG], k2[NG];
 #pragma unroll
-        for (int g = 0; g < NG; ++g) kg[g] = prm[g * NL + L0].k;
+        for (int g = 0; g < NG; ++g) {
+            kg[g] = prm[g * NL + L0].k;
+            k1[g] = (float)kg[g];
+            k2[g] = (float)(kg[g] - (double)k1[g]);
+        }
 #pragma unroll
         for (int g = 0; g < NG; ++g) rg[g] = pw(g) == kRealPower ? kg[g] / kg[BASE] : 1.0;
         const float *A = ba.cost + pd.a_off;
-        const double *h = ba.other + pd.other_off;
         const float *B = ba.fac_other + pd.other_off;
+        const float *PS = ba.psi_other + pd.other_off;
+        const float *PM = ba.m_other + pd.other_off;
         double *pot = ba.pot + pd.pot_off;
         const int ncols4 = (pd.ncols + 3) & ~3;  // potentials / factors are padded to here
         const int nchunks = (ncols4 + CH - 1) / CH;
         const float *rowp[WROWS];
-        double fk[WROWS][NG];
+        float nrow[WROWS][NG];  // integer row shifts of the FP32-split groups
+        double fkb[WROWS];      // REAL modes: the base's clamped FP64 row shift
         float s[WROWS][E];
 #pragma unroll
         for (int r = 0; r < WROWS; ++r) {
             const int row = min(tl.row0 + wave * WROWS + r, pd.nrows - 1);
             rowp[r] = A + (long long)row * pd.lda;
+            fkb[r] = 0.0;
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
-                const int gr = pw(g) ? BASE : g;  // whose reference potential shifts this group's rows
-                fk[r][g] = pot[(long long)(gr * NL + L0) * pd.pot_ld + row] * kg[g];
+                nrow[r][g] = 0.f;
+                if (split32(g)) {
+                    float ps_;
+                    split_shift(pot[(long long)(g * NL + L0) * pd.pot_ld + row] * kg[g], &ps_, &nrow[r][g]);
+                }
+                if (full64(g))
+                    fkb[r] = fmin(fmax(pot[(long long)(g * NL + L0) * pd.pot_ld + row] * kg[g], -kShiftClamp), kShiftClamp);
 #pragma unroll
                 for (int l = 0; l < NL; ++l) s[r][g * NL + l] = 0.f;
             }
         }
         // staging: thread t holds columns t + p * THREADS of a chunk (zero-mass
         // padding beyond ncols4); loaded into registers one chunk ahead
-        double hv[NH][PER];
+        float psv[NX][PER], pmv[NX][PER];
         float bv[NBS][PER];
 #define WOTK_LDS_FETCH(k)                                                                        \
     _Pragma("unroll") for (int p = 0; p < PER; ++p) {                                           \
@@ -656,15 +711,19 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
         const int col_ = (k) * CH + j_;                                                         \
         const bool ok_ = j_ < CH && col_ < ncols4;                                              \
         _Pragma("unroll") for (int g = 0; g < NG; ++g) {                                        \
-            if (pw(g)) {                                                                        \
-                _Pragma("unroll") for (int l = 0; l < NL; ++l)                                  \
-                    bv[bslot(g) + l][p] = ok_ ? B[(long long)(g * NL + l) * pd.other_ld + col_] : 0.f; \
-            } else {                                                                            \
-                hv[hslot(g)][p] = ok_ ? h[(long long)(g * NL + L0) * pd.other_ld + col_] : -INFINITY; \
-                _Pragma("unroll") for (int i = 0; i < NB; ++i) {                                \
+            if (staged(g)) {                                                                    \
+                const long long at_ = (long long)(g * NL + L0) * pd.other_ld + col_;            \
+                psv[xslot(g)][p] = ok_ ? PS[at_] : 0.f;                                         \
+                pmv[xslot(g)][p] = ok_ ? PM[at_] : -(float)kShiftClamp;                         \
+            }                                                                                   \
+            if (pw(g) == 0) {                                                                   \
+                _Pragma("unroll") for (int i = 0; i < NL - 1; ++i) {                            \
                     const int l_ = i < L0 ? i : i + 1;                                          \
                     bv[bslot(g) + i][p] = ok_ ? B[(long long)(g * NL + l_) * pd.other_ld + col_] : 0.f; \
                 }                                                                               \
+            } else {                                                                            \
+                _Pragma("unroll") for (int l = 0; l < NL; ++l)                                  \
+                    bv[bslot(g) + l][p] = ok_ ? B[(long long)(g * NL + l) * pd.other_ld + col_] : 0.f; \
             }                                                                                   \
         }                                                                                       \
     }
@@ -672,7 +731,10 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
     _Pragma("unroll") for (int p = 0; p < PER; ++p) {                                           \
         const int j_ = tid + p * THREADS;                                                       \
         if (j_ < CH) {                                                                          \
-            _Pragma("unroll") for (int x = 0; x < NH; ++x) h_lds[buf][x][j_] = hv[x][p];        \
+            _Pragma("unroll") for (int x = 0; x < NX; ++x) {                                    \
+                psi_lds[buf][x][j_] = psv[x][p];                                                \
+                m_lds[buf][x][j_] = pmv[x][p];                                                  \
+            }                                                                                   \
             _Pragma("unroll") for (int x = 0; x < NBS; ++x) b_lds[buf][x][j_] = bv[x][p];       \
         }                                                                                       \
     }
@@ -698,23 +760,14 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
                 for (int r = 0; r < WROWS; ++r) cnext[r] = bload4(rowp[r] + (k + 1) * CH + cl);
             }
             if (c < ncols4) {
-                double cv[WROWS][4];
-#pragma unroll
-                for (int r = 0; r < WROWS; ++r) {
-                    cv[r][0] = ccur[r].x;
-                    cv[r][1] = ccur[r].y;
-                    cv[r][2] = ccur[r].z;
-                    cv[r][3] = ccur[r].w;
-                }
-                // the base group first: derived groups take powers of its terms, or
+                // the base group first: the others take powers of its terms, or
                 // multiples of its exponents
-                constexpr bool REAL = pw(0) == kRealPower || pw(1) == kRealPower;
                 float tb[WROWS][4];
                 double yb[WROWS][REAL ? 4 : 1];
 #pragma unroll
                 for (int gi = 0; gi < NG; ++gi) {
                     const int g = NG - 1 - gi;
-                    if (pw(g)) {
+                    if (!staged(g)) {
                         float4 bf[NL];
 #pragma unroll
                         for (int l = 0; l < NL; ++l) bf[l] = *reinterpret_cast<const float4 *>(&b_lds[buf][bslot(g) + l][cl]);
@@ -733,37 +786,49 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
                                 s[r][g * NL + l] = fmaf(tq[3], bf[l].w, fmaf(tq[2], bf[l].z, fmaf(tq[1], bf[l].y, fmaf(tq[0], bf[l].x, s[r][g * NL + l]))));
                         }
                     } else {
-                        const double2 h01 = *reinterpret_cast<const double2 *>(&h_lds[buf][hslot(g)][cl]);
-                        const double2 h23 = *reinterpret_cast<const double2 *>(&h_lds[buf][hslot(g)][cl + 2]);
-                        const double hk[4] = {h01.x * kg[g], h01.y * kg[g], h23.x * kg[g], h23.y * kg[g]};
-                        float4 bf[NB];
+                        const float4 ps4 = *reinterpret_cast<const float4 *>(&psi_lds[buf][xslot(g)][cl]);
+                        const float4 pm4 = *reinterpret_cast<const float4 *>(&m_lds[buf][xslot(g)][cl]);
+                        const float psq[4] = {ps4.x, ps4.y, ps4.z, ps4.w};
+                        const float pmq[4] = {pm4.x, pm4.y, pm4.z, pm4.w};
+                        const int nf = pw(g) == 0 ? NL - 1 : NL;  // factor rows of the group
+                        float4 bf[NL];
 #pragma unroll
-                        for (int i = 0; i < NB; ++i) bf[i] = *reinterpret_cast<const float4 *>(&b_lds[buf][bslot(g) + i][cl]);
+                        for (int i = 0; i < NL; ++i)
+                            if (i < nf) bf[i] = *reinterpret_cast<const float4 *>(&b_lds[buf][bslot(g) + i][cl]);
 #pragma unroll
                         for (int r = 0; r < WROWS; ++r) {
+                            const float cq[4] = {ccur[r].x, ccur[r].y, ccur[r].z, ccur[r].w};
                             float tq[4];
-                            if (REAL && g == BASE) {
+                            if (full64(g)) {
+                                // y = H_j + F_i - k c in FP64 (clamped shifts; y >= -16384 as in bexp2_split)
 #pragma unroll
                                 for (int q = 0; q < 4; ++q) {
-                                    yb[r][REAL ? q : 0] = fmax(fma(-kg[g], cv[r][q], hk[q] + fk[r][g]), -16384.0);
-                                    tq[q] = bexp2_split_nc(yb[r][REAL ? q : 0]);
+                                    const double y = fmax(fma(-kg[g], (double)cq[q], ((double)pmq[q] + (double)psq[q]) + fkb[r]), -16384.0);
+                                    yb[r][REAL ? q : 0] = y;
+                                    tq[q] = bexp2_split_nc(y);
                                 }
                             } else {
 #pragma unroll
-                                for (int q = 0; q < 4; ++q) tq[q] = bexp2_split(fma(-kg[g], cv[r][q], hk[q] + fk[r][g]));
+                                for (int q = 0; q < 4; ++q) tq[q] = split_exp_f32(cq[q], k1[g], k2[g], psq[q], pmq[q] + nrow[r][g]);
                             }
                             if (g == BASE) {
 #pragma unroll
                                 for (int q = 0; q < 4; ++q) tb[r][q] = tq[q];
                             }
-                            s[r][g * NL + L0] += (tq[0] + tq[1]) + (tq[2] + tq[3]);
-                            int i = 0;
+                            if (pw(g) == 0) {
+                                s[r][g * NL + L0] += (tq[0] + tq[1]) + (tq[2] + tq[3]);
+                                int i = 0;
 #pragma unroll
-                            for (int l = 0; l < NL; ++l)
-                                if (l != L0) {
-                                    s[r][g * NL + l] = fmaf(tq[3], bf[i].w, fmaf(tq[2], bf[i].z, fmaf(tq[1], bf[i].y, fmaf(tq[0], bf[i].x, s[r][g * NL + l]))));
-                                    ++i;
-                                }
+                                for (int l = 0; l < NL; ++l)
+                                    if (l != L0) {
+                                        s[r][g * NL + l] = fmaf(tq[3], bf[i].w, fmaf(tq[2], bf[i].z, fmaf(tq[1], bf[i].y, fmaf(tq[0], bf[i].x, s[r][g * NL + l]))));
+                                        ++i;
+                                    }
+                            } else {
+#pragma unroll
+                                for (int l = 0; l < NL; ++l)
+                                    s[r][g * NL + l] = fmaf(tq[3], bf[l].w, fmaf(tq[2], bf[l].z, fmaf(tq[1], bf[l].y, fmaf(tq[0], bf[l].x, s[r][g * NL + l]))));
+                            }
                         }
                     }
                 }
@@ -778,7 +843,7 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
             __syncthreads();
         }
         // wave sums (FP32 tree over 64 lanes); lane l < RE takes (row l / E, combination l % E)
-        double S = 0.0, fkl = 0.0;
+        double S = 0.0;
 #pragma unroll
         for (int r = 0; r < WROWS; ++r)
 #pragma unroll
@@ -786,15 +851,39 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
                 float v = s[r][e];
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if (lane == r * E + e) {
-                    S = (double)v;
-                    fkl = fk[r][e / NL];
-                }
+                if (lane == r * E + e) S = (double)v;
             }
         const int my_r = lane / E, my_e = lane % E;
         const int my_row = tl.row0 + wave * WROWS + my_r;
         const bool mine = lane < RE && my_row < pd.nrows;
-        if (mine && !factored_sum_ok(S, pd.ncols, ba.fmax_other[pd.pair * E + my_e]))
+        // l2 = log2 sum_j 2^((h^e_j - C_ij) k_e) from the row's shift (its old
+        // potential, read before any potential of the tile is rewritten), and
+        // the sum with the row's fractional shift for the range check
+        double l2 = 0.0, S_true = 0.0;
+        if (lane < RE) {
+            const int g = my_e / NL, prow = min(my_row, pd.nrows - 1);
+            const int m = derived_power(MODE, g, NG);
+            const double fb = pot[(long long)(BASE * NL + L0) * pd.pot_ld + prow];
+            // mu: the part of the row's shift F left out of the terms (its fraction
+            // with the FP32 split, whose terms carry rint(clamped F); the clamp's
+            // excess in REAL modes, whose FP64 exponents carry the clamped F)
+            double mu, fk;
+            if (m == 0) {
+                const double F = pot[(long long)(g * NL + L0) * pd.pot_ld + prow] * kg[g];
+                const double Fc = fmin(fmax(F, -kShiftClamp), kShiftClamp);
+                mu = F - (REAL && g == BASE ? Fc : rint(Fc));
+                fk = F;
+            } else {
+                const double Fb = fb * kg[BASE];
+                const double Fbc = fmin(fmax(Fb, -kShiftClamp), kShiftClamp);
+                const double mub = Fb - (REAL ? Fbc : rint(Fbc));
+                mu = (m == kRealPower ? rg[g] : (double)m) * mub;
+                fk = fb * kg[g];
+            }
+            S_true = S * exp2(mu);
+            l2 = log2(S) + mu - fk;
+        }
+        if (mine && !factored_sum_ok(S_true, pd.ncols, ba.fmax_other[pd.pair * E + my_e]))
             atomicOr(&bad_row[wave * WROWS + my_r], 1 << my_e);
         __syncthreads();
         // combinations out of range go to the repair kernel (whole groups when the
@@ -803,8 +892,7 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
         const int m = lane < RE ? widen_to_groups<NG, NL>(bad_row[wave * WROWS + my_r], nmode) : 0;
         const bool row_ok = mine && !((m >> my_e) & 1);
         if (lane < RE) {
-            const double p =
-                new_potential(ba, pd, prm[my_e], min(my_row, pd.nrows - 1), tl.side_pair, log2(S) - fkl, my_e);
+            const double p = new_potential(ba, pd, prm[my_e], min(my_row, pd.nrows - 1), tl.side_pair, l2, my_e);
             pot_sh[wave][lane] = p;
             if (row_ok) pot[(long long)my_e * pd.pot_ld + my_row] = p;
             if (mine && m && my_e == 0) queue_row(ba, tl.side_pair, my_row, m);
@@ -816,6 +904,11 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
             ba.fac[pd.pot_off + (long long)my_e * pd.pot_ld + my_row] =
                 factor_of(nxt[gref].k, pot_sh[wave][lane], pot_sh[wave][my_r * E + factor_ref(nmode, my_e, NL, NG)], &dpos);
             atomic_max_nonneg(&fmax_sh[my_e], dpos);
+            if (my_e == gref) {  // the group's column shift for the reading sweep
+                const int sr = derived_power(nmode, my_e / NL, NG) == 0 ? my_e : BASE * NL + L0;
+                split_shift(pot_sh[wave][my_r * E + sr] * nxt[gref].k, &ba.psi[pd.pot_off + (long long)my_e * pd.pot_ld + my_row],
+                            &ba.msh[pd.pot_off + (long long)my_e * pd.pot_ld + my_row]);
+            }
         }
         __syncthreads();
     }
@@ -864,11 +957,17 @@ __global__ __launch_bounds__(THREADS) void batch_repair_kernel(BatchArgs ba) {
         }
         __syncthreads();
         if (ba.group && tid < E && ((mask >> tid) & 1)) {
-            const int ref = factor_ref(nxt[0].share, tid, ba.group, E / ba.group);
+            const int ng = E / ba.group;
+            const int ref = factor_ref(nxt[0].share, tid, ba.group, ng);
             const int gref = (tid / ba.group) * ba.group + ba.group / 2;  // k of the combination's own group
             float dpos;
             ba.fac[pd.pot_off + (long long)tid * pd.pot_ld + row] = factor_of(nxt[gref].k, pot_sh[tid], pot_sh[ref], &dpos);
             atomic_max_nonneg(&ba.fmax[pd.pair * E + tid], dpos);
+            if (tid == gref) {  // the group's column shift (the LDS-staged kernel reads it)
+                const int sr = derived_power(nxt[0].share, tid / ba.group, ng) == 0 ? tid : (ng - 1) * ba.group + ba.group / 2;
+                split_shift(pot_sh[sr] * nxt[gref].k, &ba.psi[pd.pot_off + (long long)tid * pd.pot_ld + row],
+                            &ba.msh[pd.pot_off + (long long)tid * pd.pot_ld + row]);
+            }
         }
         __syncthreads();
     }
@@ -937,6 +1036,20 @@ __global__ void batch_pad_kernel(double *__restrict__ pot, const PairDev *__rest
     }
 }
 
+// column shifts of the LDS-staged kernel: potentials start at 0 (shift 0),
+// padding rows [n, n_pad) at -2^22 (their terms vanish)
+__global__ void batch_shift_init_kernel(float *__restrict__ psi, float *__restrict__ msh,
+                                        const PairDev *__restrict__ pairs, int n_pairs, int E) {
+    const int p = blockIdx.x;
+    if (p >= n_pairs) return;
+    const PairDev pd = pairs[p];
+    for (int t = threadIdx.x; t < E * pd.pot_ld; t += blockDim.x) {
+        const int i = t % pd.pot_ld;
+        psi[pd.pot_off + t] = 0.f;
+        msh[pd.pot_off + t] = i < pd.nrows ? 0.f : -(float)kShiftClamp;
+    }
+}
+
 // factors of the shared-exponential kernel: 1 on the rows (potentials start
 // equal across combinations), 0 on the padding rows [n, n_pad)
 __global__ void batch_fac_init_kernel(float *__restrict__ fac, const PairDev *__restrict__ pairs, int n_pairs, int E) {
@@ -952,6 +1065,7 @@ __global__ void batch_fac_init_kernel(float *__restrict__ fac, const PairDev *__
 struct BatchWork {
     double *f, *g, *logw, *logq, *lognf, *logng;
     float *ff, *fg;  // column factors of the shared-exponential kernel
+    float *psf, *msf, *psg, *msg;  // column shifts of the LDS-staged kernel
     IterParam *table;
     PairDev *pairs_f, *pairs_g;
     Tile *tiles_f, *tiles_g;
@@ -981,6 +1095,10 @@ static size_t batch_carve(Arena &ar, const wotk_batch_pair *pairs, int n_pairs, 
     w->g = ar.take<double>((size_t)ng * E);
     w->ff = ar.take<float>((size_t)nf * E);
     w->fg = ar.take<float>((size_t)ng * E);
+    w->psf = ar.take<float>((size_t)nf * E);
+    w->msf = ar.take<float>((size_t)nf * E);
+    w->psg = ar.take<float>((size_t)ng * E);
+    w->msg = ar.take<float>((size_t)ng * E);
     w->logw = ar.take<double>((size_t)nlw * E);
     w->logq = ar.take<double>((size_t)n_pairs * E);
     w->lognf = ar.take<double>(n_pairs);
@@ -1371,6 +1489,8 @@ extern "C" int wotk_batch_begin(const float *cost, const wotk_batch_pair *pairs,
     batch_pad_kernel<<<n_pairs, 64, 0, s>>>(w.g, w.pairs_g, n_pairs, E);
     batch_fac_init_kernel<<<n_pairs, 256, 0, s>>>(w.ff, w.pairs_f, n_pairs, E);
     batch_fac_init_kernel<<<n_pairs, 256, 0, s>>>(w.fg, w.pairs_g, n_pairs, E);
+    batch_shift_init_kernel<<<n_pairs, 256, 0, s>>>(w.psf, w.msf, w.pairs_f, n_pairs, E);
+    batch_shift_init_kernel<<<n_pairs, 256, 0, s>>>(w.psg, w.msg, w.pairs_g, n_pairs, E);
     WOTK_LAUNCH_CHECK();
     batch_growth_kernel<<<dim3(n_pairs, E), 256, 0, s>>>(G, w.g_off, w.n_src, w.logw, w.lw_off, w.logq, E,
                                                          growth_per_combo ? 1 : 0);
@@ -1383,10 +1503,10 @@ extern "C" int wotk_batch_begin(const float *cost, const wotk_batch_pair *pairs,
     const int64_t n_tf = (int64_t)tf.size(), n_tg = (int64_t)tg.size();  // <= ntf, ntg (one-row tiles)
     run->af = BatchArgs{cost, w.pairs_f, w.tiles_f, (int)n_tf, w.table, w.iter, E, n_pairs, 0, w.f, w.g, w.logw,
                         w.logq, w.lognf, w.queues, w.ff, w.fg, 0, w.exact_rows, w.bad_f, w.queues + 2, group,
-                        w.fmax_f, w.fmax_g, w.fmax_g, n_pairs * E};
+                        w.fmax_f, w.fmax_g, w.fmax_g, n_pairs * E, w.psf, w.msf, w.psg, w.msg};
     run->ag = BatchArgs{cost, w.pairs_g, w.tiles_g, (int)n_tg, w.table, w.iter, E, n_pairs, 1, w.g, w.f, w.logw,
                         w.logq, w.logng, w.queues + 1, w.fg, w.ff, 1, w.exact_rows, w.bad_g, w.queues + 3, group,
-                        w.fmax_g, w.fmax_f, w.fmax_f, n_pairs * E};
+                        w.fmax_g, w.fmax_f, w.fmax_f, n_pairs * E, w.psg, w.msg, w.psf, w.msf};
     const int grid = persistent_grid(*variant);
     run->grid_f = (int)std::min<int64_t>(grid, n_tf);
     run->grid_g = (int)std::min<int64_t>(grid, n_tg);

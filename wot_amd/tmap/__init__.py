@@ -6,9 +6,10 @@
 * ``Population``, ``unique_timepoint``, ``find_path`` (wot/population.py,
   wot/tmap/util.py:52-72, wot/tmap/chaining.py:43-110).
 * Long-range couplings (glue_transport_maps / chain_transport_maps,
-  wot/tmap/util.py:74-94 and wot/tmap/chaining.py:4-40): maps multiplied in
-  FP32 on the matrix cores (``wotk_gemm_f32``) and kept in HBM between
-  products; as in the reference there is no normalisation.
+  wot/tmap/util.py:74-94 and wot/tmap/chaining.py:4-40): maps multiplied on
+  the device in FP64 like the reference's np.dot (rocBLAS DGEMM), or in FP32 on
+  the matrix cores (``wotk_gemm_f32``, ``precision='float32'``; ~1e-7
+  relative); as in the reference there is no normalisation.
 """
 import ctypes
 
@@ -48,19 +49,30 @@ def chain_product(maps):
     return out if isinstance(out, torch.Tensor) else torch.from_numpy(np.asarray(out, np.float32)).cuda()
 
 
-def glue_transport_maps(tmap_0, tmap_1):
+def glue_transport_maps(tmap_0, tmap_1, precision='float64'):
     """Transport map t0 -> t2 from t0 -> t1 and t1 -> t2 (util.py:74-94): rows
     of tmap_1 re-ordered to tmap_0's columns, then multiplied (no column
-    normalisation, as in the reference)."""
+    normalisation, as in the reference).  ``precision``: 'float64' (the
+    reference's arithmetic, rocBLAS DGEMM on the device) or 'float32'
+    (``wotk_gemm_f32`` on the matrix cores)."""
+    if precision not in ('float64', 'float32'):
+        raise ValueError("precision must be 'float64' or 'float32'")
     idx = tmap_1.obs.index.get_indexer_for(tmap_0.var.index)
     x1 = np.asarray(tmap_1.X)[idx, :]
-    res = gemm_f32(np.asarray(tmap_0.X, np.float32), x1.astype(np.float32)).cpu().numpy().astype(np.float64)
+    if precision == 'float32':
+        res = gemm_f32(np.asarray(tmap_0.X, np.float32), x1.astype(np.float32)).cpu().numpy().astype(np.float64)
+    else:
+        torch = _dev.torch_mod()
+        a = torch.from_numpy(np.ascontiguousarray(tmap_0.X, dtype=np.float64)).cuda()
+        b = torch.from_numpy(np.ascontiguousarray(x1, dtype=np.float64)).cuda()
+        res = (a @ b).cpu().numpy()
     return _anndata.AnnData(res, tmap_0.obs.copy(), tmap_1.var.copy())
 
 
-def chain_transport_maps(tmap_model, pairs_list):
+def chain_transport_maps(tmap_model, pairs_list, precision='float64'):
     """Chain the maps of consecutive pairs (chaining.py:4-40); ``tmap_model``
-    needs ``get_coupling(t0, t1)`` returning an AnnData-like map."""
+    needs ``get_coupling(t0, t1)`` returning an AnnData-like map;
+    ``precision`` as for glue_transport_maps."""
     pairs_list = list(pairs_list)
     for i in range(len(pairs_list) - 1):
         if pairs_list[i][1] != pairs_list[i + 1][0]:
@@ -71,5 +83,5 @@ def chain_transport_maps(tmap_model, pairs_list):
     tmap_0 = tmap_model.get_coupling(*pairs_list.pop(0))
     while len(pairs_list) > 0:
         tmap_1 = tmap_model.get_coupling(*pairs_list.pop(0))
-        tmap_0 = glue_transport_maps(tmap_0, tmap_1)
+        tmap_0 = glue_transport_maps(tmap_0, tmap_1, precision=precision)
     return tmap_0
