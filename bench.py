@@ -249,6 +249,8 @@ def bench_timecourse(args):
     if args.iters is not None:
         sched['extra_iter'] = sched['extra_iter'] * args.iters // sched['scaling_iter']
         sched['scaling_iter'] = args.iters
+    if args.inner_iter_max is not None:
+        sched['inner_iter_max'] = args.inner_iter_max
     n_iter = sched['scaling_iter'] + sched['extra_iter']
     step_iters = min(STEP_ITERS, n_iter)
     steps_per_course = -(-n_iter // step_iters)
@@ -1262,6 +1264,9 @@ def main():
                     help='timecourse: force the shared-exponential sweep tile ROWSxTHREADS (no tuning)')
     ap.add_argument('--no-parity', action='store_true',
                     help='timecourse: skip the parity and cost-error legs (profiling runs)')
+    ap.add_argument('--inner-iter-max', type=int, default=None,
+                    help='timecourse: override inner_iter_max (with --iters, a short schedule whose eps converges: '
+                         'counter runs of every sweep mode)')
     ap.add_argument('--share-eps', default='on', choices=['on', 'off'],
                     help='timecourse: eps groups in integer k ratios take powers of the largest eps\'s '
                          'exponential once the schedule has converged')
