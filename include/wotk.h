@@ -176,15 +176,15 @@ int wotk_plan(const float *C, int64_t ldc, int64_t I, int64_t J, const double *f
  * are recomputed exactly.  Ignored (the per-combination kernel runs) when the
  * grid has no such grouping.
  * flags & WOTK_BATCH_SHARE_EPS (with WOTK_BATCH_SHARE_EXP, three groups of
- * three): groups 0 and 1 reuse the last group's FP64 exponent.  At every
- * iteration where k = log2(e)/eps of a group is an integer multiple m >= 1 of
- * the last group's (within 1e-10 relative, for every pair) -- e.g. eps
- * {0.01, 0.05, 0.1} once the eps schedule has converged -- it takes the last
- * group's exponential to the power m (FP32 products) instead of its own;
- * otherwise its exponent is r = k_g / k_last times the last group's (ratios up
- * to 2^16), with its own exponential.  Compiled (group 1, group 0): (2, 10),
- * (2, 4), (2, r), (1, 1) and (r, r).  wotk_batch_modes reports the mode per
- * iteration.
+ * three, on the LDS-staged tiles; ignored otherwise): groups 0 and 1 reuse the
+ * last group's exponent.  At every iteration where k = log2(e)/eps of a group
+ * is an integer multiple m >= 1 of the last group's (within 1e-10 relative,
+ * for every pair) -- e.g. eps {0.01, 0.05, 0.1} once the eps schedule has
+ * converged -- it takes the last group's exponential to the power m (FP32
+ * products) instead of its own; otherwise its exponent is r = k_g / k_last
+ * times the last group's (ratios up to 2^16), with its own exponential.
+ * Compiled (group 1, group 0): (2, 10), (2, 4), (2, r), (1, 1) and (r, r).
+ * wotk_batch_modes reports the mode per iteration.
  * stats: sweeps / sweep_ms as for the single solver; stabilisations = rows
  * recomputed exactly (fixed shift or factors out of range); gap_checks =
  * combinations per shared-exponential group (0: one exponential per
