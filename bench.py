@@ -292,7 +292,7 @@ def bench_timecourse(args):
         torch.cuda.empty_cache()
     torch.cuda.synchronize()
     plan_buf = torch.empty(max(pair_ij), dtype=torch.float64, device='cuda')
-    sweep_ms, sweeps, exact_rows, shared, derived_iters = 0.0, 0, 0, None, None
+    sweep_ms, sweeps, exact_rows, shared, derived_iters, modes = 0.0, 0, 0, None, None, None
     res = tc = run = None
     courses = 0
     t0 = time.perf_counter()
@@ -303,7 +303,8 @@ def bench_timecourse(args):
         left = run.advance(step_iters)
         if left == 0:
             res = run.result()
-            derived_iters = int((run.modes() != 0).sum())
+            modes = run.modes()
+            derived_iters = int((modes != 0).sum())
             for p in range(len(pair_ij)):
                 n_i, n_j = res.shapes[p]
                 for e in range(n_combo):
@@ -333,6 +334,14 @@ def bench_timecourse(args):
     # (one read shared by all combinations) and one exp per (entry, combination)
     alg_bytes = 4.0 * sum_ij
     achieved = alg_bytes / avg_sweep / 1e9
+    # exponentials per cost entry over the course's sweep modes (wotk_batch_modes): one per eps
+    # group on its own or real-ratio exponent, none for an integer power of the base's
+    mode_hist, exps_per_entry = None, float(len(GRID['epsilons']))
+    if modes is not None and len(modes):
+        def n_exp(m):
+            return len(GRID['epsilons']) if m == 0 else 1 + int((m & 0xff) == 255) + int((m >> 8) == 255)
+        mode_hist = {str(int(m)): int((modes == m).sum()) for m in np.unique(modes)}
+        exps_per_entry = float(np.mean([n_exp(int(m)) for m in modes]))
     rates = valu_rates(torch)
     entries_per_s = sum_ij / avg_sweep
     out = {
@@ -362,12 +371,15 @@ def bench_timecourse(args):
                      'traffic_source': 'profiles/batch_shared_lds_kernel_timecourse_38pairs_9combos.traffic.json '
                                        '(calibrated FETCH_SIZE + WRITE_SIZE, not measured in this run)',
                      'compute': {'note': 'the sweep is VALU-issue-bound, not HBM-bound: one FP32 cost read '
-                                         'serves 9 combinations',
+                                         'serves 9 combinations; SQ counters put the VALU busy ~100 % in every '
+                                         'sweep mode (profiles/batch_shared_lds_kernel_timecourse_38pairs_9combos.md)',
                                  'entries_per_s': entries_per_s,
                                  'entry_combinations_per_s': entries_per_s * n_combo,
-                                 'exp_f32_per_s': entries_per_s * len(GRID['epsilons']),
+                                 'exp_f32_per_entry_course_average': exps_per_entry,
+                                 'exp_f32_per_s': entries_per_s * exps_per_entry,
                                  'measured_exp_f32_peak_per_s': rates['exp_f32'],
-                                 'exp_frac': entries_per_s * len(GRID['epsilons']) / rates['exp_f32'],
+                                 'exp_frac': entries_per_s * exps_per_entry / rates['exp_f32'],
+                                 'sweep_modes': mode_hist,
                                  'measured_rates_per_s': rates}},
     }
     if args.no_parity:

@@ -224,3 +224,33 @@ def test_shared_eps_modes_follow_the_schedule(batch):
                 assert (m[:50] == 1 | (1 << 8)).all() and (m[50:] == REAL | (REAL << 8)).all()
             else:
                 assert (m == 0).all()
+
+
+def test_shared_eps_per_pair_grids(batch):
+    """Per-pair eps (a day-pair configuration): pair 0 on {0.01, 0.05, 0.1},
+    pair 1 on {0.02, 0.05, 0.1}.  Group 1 has the power 2 in both, group 0
+    has 10 in one and 5 in the other, so once the schedule has converged the
+    common mode is (2, real); every plan meets the oracle gate."""
+    sched = dict(scaling_iter=300, extra_iter=100, inner_iter_max=5)
+    coords, growth = _course(13, [140, 230, 190])
+    lams = (0.5, 1.0, 5.0)
+    grids = [(0.01, 0.05, 0.1), (0.02, 0.05, 0.1)]
+    eps = np.array([[e for e in g for _ in lams] for g in grids])
+    l1 = np.array([[l for _ in g for l in lams] for g in grids])
+    l2 = np.full_like(eps, 50.0)
+    tc = batch.prepare_time_course(coords, growth)
+    with batch.TimeCourseRun(tc, eps, l1, l2, epsilon0=1.0, graph_iters=50, share_eps=True, **sched) as run:
+        run.advance(run.n_iter)
+        modes = run.modes()
+        res = run.result()
+    assert (modes[:5] == 1 | (1 << 8)).all()
+    assert (modes[250:] == 2 | (REAL << 8)).all(), np.unique(modes[250:])
+    for p in range(2):
+        C = O.default_cost_matrix(coords[p], coords[p + 1])
+        for e in range(9):
+            ep, lam1 = eps[p, e], l1[p, e]
+            R = res.plan(p, e)
+            Rref = O.transport_stablev2(C, lam1, 50.0, ep, G=growth[p], epsilon0=1.0, tau=1e4, **sched)
+            err = np.linalg.norm(R - Rref) / np.linalg.norm(Rref)
+            marg = max(np.abs(R.sum(a) - Rref.sum(a)).max() / np.abs(Rref.sum(a)).max() for a in (0, 1))
+            assert err <= 1e-5 and marg <= 1e-6, (p, ep, lam1, err, marg)
