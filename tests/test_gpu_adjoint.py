@@ -18,8 +18,11 @@ def mods():
     return torch, ot, adjoint, device
 
 
-def _autograd_reference(torch, X, Y, lg, f, g, eps, lam1, lam2, mid_idx, dt=1.0):
-    """FP64 torch autograd of primal() with R = exp((f + g - C)/eps) held fixed."""
+def _autograd_reference(torch, X, Y, lg, f, g, eps, lam1, lam2, mid_idx, dt=1.0, c_values=None):
+    """FP64 torch autograd of primal() with R = exp((f + g - C)/eps) held fixed.
+    ``c_values``: take the cost's values from this matrix (the device's FP32
+    cost, widened), also where the median's derivative multiplies them, while
+    the gradient still flows through D and the median."""
     x = X.clone().requires_grad_(True)
     l = lg.clone().requires_grad_(True)
     D = (x * x).sum(1)[:, None] + (Y * Y).sum(1)[None, :] - 2.0 * x @ Y.T
@@ -32,7 +35,11 @@ def _autograd_reference(torch, X, Y, lg, f, g, eps, lam1, lam2, mid_idx, dt=1.0)
         m = flat[mid_idx[0]]
     else:
         m = 0.5 * (flat[mid_idx[0]] + flat[mid_idx[1]])
-    C = D / m
+    if c_values is None:
+        C = D / m
+    else:  # value c_values; dC/dD = 1/m and dC/dm = -c_values/m, as the kernel takes them
+        md = m.detach()
+        C = c_values + (D - D.detach()) / md - c_values * (m - md) / md
     n_i, n_j = C.shape
     p = torch.exp(dt * l)
     q = p.mean()
@@ -71,6 +78,44 @@ def test_adjoint_kernel_vs_fp64_autograd(mods, n_i, n_j, d, solver):
     # the FP32 cost entries enter both sides identically except through C in
     # the reference (recomputed in FP64 here): bound by the FP32 rounding of C
     assert ex < 1e-5 and eg < 1e-6
+
+
+@pytest.mark.parametrize('n_i,n_j,d,eps,odd_ld', [
+    (300, 413, 40, 0.05, True),    # three 16-wide coordinate tiles, ldc = 414 (scalar cost loads)
+    (257, 300, 64, 0.05, False),   # four tiles, a ragged last row block
+    (100, 5000, 30, 0.05, False),  # 20 column splits, the last one ragged
+    (200, 250, 70, 0.05, False),   # d > 64: the LDS kernel
+    (300, 400, 30, 0.002, False),  # exp(-C/eps) below 2^-1000 for ~9% of the entries: R formed directly
+])
+def test_adjoint_shapes_against_autograd(mods, n_i, n_j, d, eps, odd_ld):
+    """Adjoint at arbitrary potentials (the partial derivative at a fixed plan
+    needs no optimum) against FP64 autograd on the same FP32 cost values."""
+    torch, ot, adjoint, device = mods
+    rng = np.random.default_rng(n_i + n_j + d)
+    X = torch.from_numpy(rng.normal(size=(n_i, d))).cuda()
+    Y = torch.from_numpy(rng.normal(size=(n_j, d)) + 0.3).cuda()
+    lg = torch.from_numpy(rng.normal(scale=0.3, size=n_i)).cuda()
+    cost = ot.DeviceCost.from_coordinates(X, Y)
+    if odd_ld:
+        C2 = torch.zeros((n_i, n_j + 1), dtype=torch.float32, device=X.device)
+        C2[:, :n_j] = cost.C[:, :n_j]
+        cost = ot.DeviceCost(C2, cost.CT, n_i, n_j, median=cost.median, median_index=cost.median_index)
+        assert cost.ldc % 4 != 0
+    Cv = cost.C[:, :n_j].double()
+    # potentials that keep every row's and column's plan mass positive
+    f = 0.5 * Cv.min(1).values
+    g = 0.5 * Cv.min(0).values
+    G = torch.exp(lg)
+    gx, glg = adjoint.adjoint(cost, X, Y, G, f, g, 1.0, 50.0, eps)
+    mid = cost.median_index.cpu().numpy()
+    _, rgx, rglg = _autograd_reference(torch, X, Y, lg, f, g, eps, 1.0, 50.0, mid, c_values=Cv)
+    assert torch.isfinite(rgx).all() and torch.isfinite(rglg).all()
+    ex = float(torch.linalg.norm(gx - rgx) / torch.linalg.norm(rgx))
+    eg = float(torch.linalg.norm(glg - rglg) / torch.linalg.norm(rglg))
+    print('adjoint %dx%d d=%d eps=%g vs autograd: grad_x %.3e grad_log_g %.3e' % (n_i, n_j, d, eps, ex, eg))
+    # same cost values on both sides: FP64 rounding only (an entry left out or
+    # misplaced would show at ~1e-3)
+    assert ex < 1e-12 and eg < 1e-12
 
 
 def test_adjoint_vs_oracle_gradient(mods):

@@ -137,6 +137,266 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_rows_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same rows pass on the matrix cores (d <= 64), split over columns.
+//
+// A wave owns RB blocks of 16 rows and the columns of one split; lane
+// (m, q) = (lane % 16, lane / 16) forms W for row 16 rb + m and columns
+// j + 4 q + s (s < 4) of each 16-column step: W is the A operand (16 x 4,
+// k = q) of four k-steps of v_mfma_f64_16x16x4f64 and Y of the same columns
+// (coordinates 16 nd + m) the B operand, so V = W Y runs on the matrix cores
+// beside the exponentials.  Every lane reads 16 contiguous bytes of each of
+// its rows per step; the next step's cost, column factors and Y are loaded
+// while the current one computes.  No LDS and no barriers.
+//
+// One exponential per entry: R = exp(-C/eps) exp(f_i/eps) exp(g_j/eps), the
+// row's factor formed once per lane and the column's by
+// adjoint_colexp_kernel, when exp(-C/eps) and the factor product both lie in
+// [2^-1000, 2^1000]; otherwise (an underflowing K~, an overflowing factor,
+// NaN) R = exp((f_i + g_j - C)/eps) directly.
+//
+// Outputs per split (summed over splits in a fixed order afterwards):
+// V (n_i x d), the row sums of R and W, and per wave sum W C.
+constexpr int kAdjMfmaE = 4;                 // columns per lane and step
+constexpr int kAdjMfmaStep = 4 * kAdjMfmaE;  // columns per step
+constexpr int kAdjMfmaWaves = 4;             // waves per workgroup
+constexpr int kAdjMfmaTargetWaves = 4096;    // splits are chosen to reach ~4 waves per SIMD
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+struct AdjMfmaArgs {
+    const float *A;
+    int64_t lda;
+    int n_i, n_j, d;
+    const double *f, *g, *bcol, *Y;
+    double inv_eps, inv_ij;
+    float c_lim;  // |C| <= c_lim: exp(-C/eps) within [2^-1000, 2^1000] (690 eps, rounded down)
+    int split_cols, n_tiles;
+    bool vec;  // 16-byte cost loads (lda and the base 16-byte aligned)
+    double *Vp, *rp, *swp, *wcp;  // [split][n_i][d], [split][n_i], [split][n_i], [split][n_tiles]
+};
+
+template <int RB, int ND>
+struct AdjStep {
+    float c[RB][kAdjMfmaE];
+    double b[kAdjMfmaE];
+    double y[kAdjMfmaE][ND];
+};
+
+// FULL: every column of the step lies before j_end (wave-uniform): no masks.
+// Otherwise indices are clamped into range and the values selected to 0, so
+// every load is in bounds and none sits under a branch.
+template <int RB, int ND, bool FULL>
+__device__ __forceinline__ void adj_fetch(AdjStep<RB, ND> &st, const AdjMfmaArgs &a, const float *const rowp[RB],
+                                          int j, int j_end, int m, int q) {
+    constexpr int E = kAdjMfmaE;
+    const int cb = j + E * q;
+    if (FULL && a.vec) {
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(rowp[rb] + cb));
+            st.c[rb][0] = v.x;
+            st.c[rb][1] = v.y;
+            st.c[rb][2] = v.z;
+            st.c[rb][3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                const int col = FULL ? cb + s : min(cb + s, j_end - 1);
+                const float c = rowp[rb][col];
+                st.c[rb][s] = FULL || cb + s < j_end ? c : 0.f;
+            }
+    }
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const bool ok = FULL || cb + s < j_end;
+        const int col = FULL ? cb + s : min(cb + s, j_end - 1);
+        const double b = a.bcol[col];
+        st.b[s] = ok ? b : 0.0;
+#pragma unroll
+        for (int nd = 0; nd < ND; ++nd) {
+            const int dd = 16 * nd + m;
+            const double y = a.Y[(int64_t)col * a.d + min(dd, a.d - 1)];
+            st.y[s][nd] = ok && dd < a.d ? y : 0.0;
+        }
+    }
+}
+
+template <int RB, int ND>
+__device__ __forceinline__ void adj_fetch_step(AdjStep<RB, ND> &st, const AdjMfmaArgs &a,
+                                               const float *const rowp[RB], int j, int j_end, int m, int q) {
+    if (j + kAdjMfmaStep <= j_end)
+        adj_fetch<RB, ND, true>(st, a, rowp, j, j_end, m, q);
+    else
+        adj_fetch<RB, ND, false>(st, a, rowp, j, j_end, m, q);
+}
+
+// Entries whose row and column factors lie in [2^-500, 2^500] and whose cost
+// has |C| <= 690 eps take R = exp(-C/eps) (a f) (b g); their exponential, W
+// and MFMAs form one basic block per step, so the matrix-core products of one
+// entry overlap the exponentials of the next.  The others (none in most
+// sweeps) enter that pass with weight 0 and are added afterwards with R formed
+// directly, under one branch per step.
+template <int RB, int ND>
+__device__ __forceinline__ void adj_compute(const AdjStep<RB, ND> &st, const AdjMfmaArgs &a, const bool row_ok[RB],
+                                            const bool row_in[RB], const double fr[RB], const double ar[RB],
+                                            float c_lim, int j, int j_end, int q, d4 acc[RB][ND],
+                                            double acc_r[RB], double acc_w[RB], double &swc) {
+    constexpr int E = kAdjMfmaE;
+    const int cb = j + E * q;
+    bool any_slow = false;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const bool colok = cb + s < j_end;
+        const bool col_in = st.b[s] >= 0x1p-500 && st.b[s] <= 0x1p500;
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            const float cf = st.c[rb][s];
+            const bool fast = row_ok[rb] && colok && row_in[rb] && col_in && fabsf(cf) <= c_lim;
+            any_slow |= row_ok[rb] && colok && !fast;
+            const double c = (double)cf;
+            const double kt = exp(-c * a.inv_eps);
+            const double R = kt * (ar[rb] * st.b[s]);
+            const double w = fast ? (R - kt) * a.inv_ij : 0.0;
+            acc_r[rb] += fast ? R : 0.0;
+            acc_w[rb] += w;
+            swc = fma(w, c, swc);
+#pragma unroll
+            for (int nd = 0; nd < ND; ++nd)
+                acc[rb][nd] = __builtin_amdgcn_mfma_f64_16x16x4f64(w, st.y[s][nd], acc[rb][nd], 0, 0, 0);
+        }
+    }
+    if (__builtin_amdgcn_read_exec() & __ballot(any_slow)) {
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const bool colok = cb + s < j_end;
+            const bool col_in = st.b[s] >= 0x1p-500 && st.b[s] <= 0x1p500;
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                const float cf = st.c[rb][s];
+                const bool slow = row_ok[rb] && colok && !(row_in[rb] && col_in && fabsf(cf) <= c_lim);
+                double R = 0.0, w = 0.0;
+                const double c = (double)cf;
+                if (slow) {
+                    R = exp((fr[rb] + a.g[cb + s] - c) * a.inv_eps);
+                    w = (R - exp(-c * a.inv_eps)) * a.inv_ij;
+                }
+                acc_r[rb] += R;
+                acc_w[rb] += w;
+                swc = fma(w, c, swc);
+#pragma unroll
+                for (int nd = 0; nd < ND; ++nd)
+                    acc[rb][nd] = __builtin_amdgcn_mfma_f64_16x16x4f64(w, st.y[s][nd], acc[rb][nd], 0, 0, 0);
+            }
+        }
+    }
+}
+
+template <int RB, int ND>
+__global__ __launch_bounds__(kAdjMfmaWaves * 64) void adjoint_mfma_kernel(AdjMfmaArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * kAdjMfmaWaves + (threadIdx.x >> 6);
+    if (tile >= a.n_tiles) return;  // whole waves only; the kernel has no barriers
+    const int split = blockIdx.y;
+    const int m = lane & 15, q = lane >> 4;
+    const int i0 = tile * 16 * RB;
+    const int j_begin = split * a.split_cols;
+    const int j_end = min(a.n_j, j_begin + a.split_cols);
+    const float *rowp[RB];
+    bool row_ok[RB], row_in[RB];
+    double fr[RB], ar[RB], acc_r[RB], acc_w[RB];
+    d4 acc[RB][ND];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        const int r = i0 + 16 * rb + m;
+        row_ok[rb] = r < a.n_i;
+        rowp[rb] = a.A + (int64_t)(row_ok[rb] ? r : 0) * a.lda;
+        fr[rb] = row_ok[rb] ? a.f[r] : 0.0;
+        ar[rb] = exp(fr[rb] * a.inv_eps);
+        row_in[rb] = ar[rb] >= 0x1p-500 && ar[rb] <= 0x1p500;
+        acc_r[rb] = 0.0;
+        acc_w[rb] = 0.0;
+#pragma unroll
+        for (int nd = 0; nd < ND; ++nd) acc[rb][nd] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    double swc = 0.0;
+    AdjStep<RB, ND> sa, sb;
+    int j = j_begin;
+    if (j < j_end) adj_fetch_step<RB, ND>(sa, a, rowp, j, j_end, m, q);
+    while (j < j_end) {  // two steps per trip: sa and sb alternate, no register copies
+        const bool has1 = j + kAdjMfmaStep < j_end;
+        if (has1) adj_fetch_step<RB, ND>(sb, a, rowp, j + kAdjMfmaStep, j_end, m, q);
+        adj_compute<RB, ND>(sa, a, row_ok, row_in, fr, ar, a.c_lim, j, j_end, q, acc, acc_r, acc_w, swc);
+        if (!has1) break;
+        const bool has2 = j + 2 * kAdjMfmaStep < j_end;
+        if (has2) adj_fetch_step<RB, ND>(sa, a, rowp, j + 2 * kAdjMfmaStep, j_end, m, q);
+        adj_compute<RB, ND>(sb, a, row_ok, row_in, fr, ar, a.c_lim, j + kAdjMfmaStep, j_end, q, acc, acc_r, acc_w, swc);
+        j += 2 * kAdjMfmaStep;
+    }
+    // D[i][n] of a 16 x 16 tile sits in lane n + 16 (i % 4) as acc[i / 4]: row q + 4 r, column m
+    double *Vp = a.Vp + (int64_t)split * a.n_i * a.d;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int nd = 0; nd < ND; ++nd)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = i0 + 16 * rb + q + 4 * r, dd = 16 * nd + m;
+                if (row < a.n_i && dd < a.d) Vp[(int64_t)row * a.d + dd] = acc[rb][nd][r];
+            }
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        double x = acc_r[rb], y = acc_w[rb];
+        x += __shfl_xor(x, 16);
+        y += __shfl_xor(y, 16);
+        x += __shfl_xor(x, 32);
+        y += __shfl_xor(y, 32);
+        const int row = i0 + 16 * rb + m;
+        if (q == 0 && row < a.n_i) {
+            a.rp[(int64_t)split * a.n_i + row] = x;
+            a.swp[(int64_t)split * a.n_i + row] = y;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) swc += __shfl_xor(swc, off);
+    if (lane == 0) a.wcp[(int64_t)split * a.n_tiles + tile] = swc;
+}
+
+// exp(g_j / eps) per column
+__global__ __launch_bounds__(256) void adjoint_colexp_kernel(const double *__restrict__ g, int n_j, double inv_eps,
+                                                             double *__restrict__ bcol) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n_j) bcol[j] = exp(g[j] * inv_eps);
+}
+
+// V, row sums of R and W: the splits' partials summed in split order
+__global__ __launch_bounds__(256) void adjoint_split_sum_kernel(const double *__restrict__ Vp,
+                                                                const double *__restrict__ rp,
+                                                                const double *__restrict__ swp, int n_split,
+                                                                int n_i, int d, double *__restrict__ V,
+                                                                double *__restrict__ r, double *__restrict__ sw) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nv = (int64_t)n_i * d;
+    if (e < nv) {
+        double x = 0.0;
+        for (int s = 0; s < n_split; ++s) x += Vp[s * nv + e];
+        V[e] = x;
+    }
+    if (e < n_i) {
+        double x = 0.0, y = 0.0;
+        for (int s = 0; s < n_split; ++s) {
+            x += rp[(int64_t)s * n_i + e];
+            y += swp[(int64_t)s * n_i + e];
+        }
+        r[e] = x;
+        sw[e] = y;
+    }
+}
+
 // scalars: [0] = S = sum W C, [1] = Q = sum_j lambda2/J (1 - y_j/q), [2] = q
 __global__ __launch_bounds__(1024) void adjoint_scalars_kernel(const double *__restrict__ partials, int nparts,
                                                                const double *__restrict__ c, int n_i, int n_j,
@@ -220,17 +480,48 @@ __global__ void adjoint_median_kernel(const double *__restrict__ X, const double
 struct AdjWork {
     double *r, *sw, *c, *V, *partials, *scal;
     int nparts;
+    // matrix-core path (d <= 64): per-split partials and the column factors
+    bool mfma;
+    int rb, nd, n_tiles, n_split, split_cols;
+    double *Vp, *rp, *swp, *bcol;
 };
 
+// matrix-core path: rows per wave, splits and columns per split of an I x J x d
+// adjoint (a function of the shape only, so the workspace size is too)
+static void adj_mfma_shape(int64_t I, int64_t J, int64_t d, AdjWork *w) {
+    w->mfma = d <= 64;
+    w->nd = (int)((d + 15) / 16);
+    w->rb = 2;
+    w->n_tiles = (int)((I + 16 * w->rb - 1) / (16 * w->rb));
+    int64_t split = (kAdjMfmaTargetWaves + w->n_tiles - 1) / w->n_tiles;
+    split = std::max<int64_t>(1, std::min<int64_t>(split, (J + 255) / 256));  // >= 256 columns per split
+    w->split_cols = (int)round_up((J + split - 1) / split, kAdjMfmaStep);
+    w->n_split = (int)((J + w->split_cols - 1) / w->split_cols);
+}
+
 static size_t adj_carve(Arena &ar, int64_t I, int64_t J, int64_t d, AdjWork *w) {
+    adj_mfma_shape(I, J, d, w);
     w->r = ar.take<double>(I);
     w->sw = ar.take<double>(I);
     w->c = ar.take<double>(J);
     w->V = ar.take<double>((size_t)I * d);
-    w->nparts = (int)((I + kAdjRows - 1) / kAdjRows);
+    w->nparts = w->mfma ? w->n_split * w->n_tiles : (int)((I + kAdjRows - 1) / kAdjRows);
     w->partials = ar.take<double>(w->nparts);
     w->scal = ar.take<double>(8);
+    w->Vp = w->rp = w->swp = w->bcol = nullptr;
+    if (w->mfma) {
+        w->Vp = ar.take<double>((size_t)w->n_split * I * d);
+        w->rp = ar.take<double>((size_t)w->n_split * I);
+        w->swp = ar.take<double>((size_t)w->n_split * I);
+        w->bcol = ar.take<double>(J);
+    }
     return ar.off;
+}
+
+template <int ND>
+static void launch_adj_mfma(const AdjMfmaArgs &a, int n_split, hipStream_t s) {
+    const dim3 grid((unsigned)((a.n_tiles + kAdjMfmaWaves - 1) / kAdjMfmaWaves), (unsigned)n_split);
+    adjoint_mfma_kernel<2, ND><<<grid, kAdjMfmaWaves * 64, 0, s>>>(a);
 }
 
 }  // namespace wotk
@@ -260,17 +551,35 @@ extern "C" int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int
     if (!workspace || !ar.ok) return fail(WOTK_EWORKSPACE, "adjoint workspace too small");
     const double inv_eps = 1.0 / epsilon;
     const double inv_ij = 1.0 / ((double)I * (double)J);
-    switch ((d + 31) / 32) {
-        case 1: adjoint_rows_kernel<1><<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps, inv_ij, Y,
-                                                                        (int)d, w.V, w.r, w.sw, w.partials); break;
-        case 2: adjoint_rows_kernel<2><<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps, inv_ij, Y,
-                                                                        (int)d, w.V, w.r, w.sw, w.partials); break;
-        case 3: adjoint_rows_kernel<3><<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps, inv_ij, Y,
-                                                                        (int)d, w.V, w.r, w.sw, w.partials); break;
-        default: adjoint_rows_kernel<4><<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps, inv_ij, Y,
-                                                                         (int)d, w.V, w.r, w.sw, w.partials); break;
+    if (w.mfma) {
+        adjoint_colexp_kernel<<<(unsigned)((J + 255) / 256), 256, 0, s>>>(g, (int)J, inv_eps, w.bcol);
+        WOTK_LAUNCH_CHECK();
+        // |C| <= c_lim keeps C/eps within 690 < 1000 ln 2 (FP32 rounding toward zero)
+        const float c_lim = std::nextafter((float)(690.0 * epsilon), 0.f);
+        AdjMfmaArgs a{C, ldc, (int)I, (int)J, (int)d, f, g, w.bcol, Y, inv_eps, inv_ij, c_lim, w.split_cols, w.n_tiles,
+                      ldc % 4 == 0 && reinterpret_cast<uintptr_t>(C) % 16 == 0, w.Vp, w.rp, w.swp, w.partials};
+        switch (w.nd) {
+            case 1: launch_adj_mfma<1>(a, w.n_split, s); break;
+            case 2: launch_adj_mfma<2>(a, w.n_split, s); break;
+            case 3: launch_adj_mfma<3>(a, w.n_split, s); break;
+            default: launch_adj_mfma<4>(a, w.n_split, s); break;
+        }
+        WOTK_LAUNCH_CHECK();
+        const int64_t n = std::max<int64_t>(I * d, I);
+        adjoint_split_sum_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(w.Vp, w.rp, w.swp, w.n_split, (int)I,
+                                                                            (int)d, w.V, w.r, w.sw);
+        WOTK_LAUNCH_CHECK();
+    } else {
+        switch ((d + 31) / 32) {
+            case 3: adjoint_rows_kernel<3><<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps,
+                                                                            inv_ij, Y, (int)d, w.V, w.r, w.sw,
+                                                                            w.partials); break;
+            default: adjoint_rows_kernel<4><<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps,
+                                                                             inv_ij, Y, (int)d, w.V, w.r, w.sw,
+                                                                             w.partials); break;
+        }
+        WOTK_LAUNCH_CHECK();
     }
-    WOTK_LAUNCH_CHECK();
     int rc = launch_plan_sums(CT, ldct, (int)J, (int)I, g, f, epsilon, w.c, s);
     if (rc) return rc;
     adjoint_scalars_kernel<<<1, 1024, 0, s>>>(w.partials, w.nparts, w.c, (int)I, (int)J, G, lambda2, w.scal);
