@@ -161,6 +161,7 @@ constexpr int kAdjMfmaE = 4;                 // columns per lane and step
 constexpr int kAdjMfmaStep = 4 * kAdjMfmaE;  // columns per step
 constexpr int kAdjMfmaWaves = 4;             // waves per workgroup
 constexpr int kAdjMfmaTargetWaves = 4096;    // splits are chosen to reach ~4 waves per SIMD
+constexpr int kAdjSuperT = 2;                // steps per super-step (each lane reads 16 T contiguous bytes of a row)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -169,7 +170,8 @@ struct AdjMfmaArgs {
     const float *A;
     int64_t lda;
     int n_i, n_j, d;
-    const double *f, *g, *bcol, *Y;
+    const double *f, *g, *bcol;
+    const double *Yp;  // Y zero-padded to 16 ND coordinates per column
     double inv_eps, inv_ij;
     float c_lim;  // |C| <= c_lim: exp(-C/eps) within [2^-1000, 2^1000] (690 eps, rounded down)
     int split_cols, n_tiles;
@@ -177,62 +179,82 @@ struct AdjMfmaArgs {
     double *Vp, *rp, *swp, *wcp;  // [split][n_i][d], [split][n_i], [split][n_i], [split][n_tiles]
 };
 
-template <int RB, int ND>
-struct AdjStep {
-    float c[RB][kAdjMfmaE];
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// Coordinates on the MFMA tiles: tiles 2p and 2p + 1 hold coordinates 32 p +
+// 2 m and 32 p + 2 m + 1 in column m (one 16-byte load per lane fetches both);
+// with ND odd the last tile holds 16 (ND - 1) + m.
+template <int ND>
+__device__ __forceinline__ int adj_coord(int nd, int m) {
+    return nd < (ND & ~1) ? 32 * (nd >> 1) + 2 * m + (nd & 1) : 16 * nd + m;
+}
+
+// column factors and Y operands of one step: columns col0 + s (s < 4) of this lane
+template <int ND>
+struct AdjYStep {
     double b[kAdjMfmaE];
     double y[kAdjMfmaE][ND];
 };
 
-// FULL: every column of the step lies before j_end (wave-uniform): no masks.
-// Otherwise indices are clamped into range and the values selected to 0, so
-// every load is in bounds and none sits under a branch.
-template <int RB, int ND, bool FULL>
-__device__ __forceinline__ void adj_fetch(AdjStep<RB, ND> &st, const AdjMfmaArgs &a, const float *const rowp[RB],
-                                          int j, int j_end, int m, int q) {
-    constexpr int E = kAdjMfmaE;
-    const int cb = j + E * q;
-    if (FULL && a.vec) {
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-            const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(rowp[rb] + cb));
-            st.c[rb][0] = v.x;
-            st.c[rb][1] = v.y;
-            st.c[rb][2] = v.z;
-            st.c[rb][3] = v.w;
-        }
-    } else {
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-            for (int s = 0; s < E; ++s) {
-                const int col = FULL ? cb + s : min(cb + s, j_end - 1);
-                const float c = rowp[rb][col];
-                st.c[rb][s] = FULL || cb + s < j_end ? c : 0.f;
-            }
+// FULL: all four columns lie before j_end (then col0 is a multiple of 4: the
+// factor loads are 16-byte aligned).  Otherwise indices are clamped into range
+// and the values selected to 0, so every load is in bounds and none sits under
+// a branch.
+template <int ND, bool FULL>
+__device__ __forceinline__ void adj_fetch_y(AdjYStep<ND> &st, const AdjMfmaArgs &a, int col0, int j_end, int m) {
+    constexpr int E = kAdjMfmaE, DP = 16 * ND;
+    if (FULL) {
+        const d2 b01 = *reinterpret_cast<const d2 *>(a.bcol + col0);
+        const d2 b23 = *reinterpret_cast<const d2 *>(a.bcol + col0 + 2);
+        st.b[0] = b01.x;
+        st.b[1] = b01.y;
+        st.b[2] = b23.x;
+        st.b[3] = b23.y;
     }
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-        const bool ok = FULL || cb + s < j_end;
-        const int col = FULL ? cb + s : min(cb + s, j_end - 1);
-        const double b = a.bcol[col];
-        st.b[s] = ok ? b : 0.0;
+        const bool ok = FULL || col0 + s < j_end;
+        const int col = FULL ? col0 + s : min(col0 + s, j_end - 1);
+        if (!FULL) {
+            const double b = a.bcol[col];
+            st.b[s] = ok ? b : 0.0;
+        }
+        const double *yr = a.Yp + (int64_t)col * DP;
 #pragma unroll
-        for (int nd = 0; nd < ND; ++nd) {
-            const int dd = 16 * nd + m;
-            const double y = a.Y[(int64_t)col * a.d + min(dd, a.d - 1)];
-            st.y[s][nd] = ok && dd < a.d ? y : 0.0;
+        for (int p = 0; p < ND / 2; ++p) {
+            const d2 v = *reinterpret_cast<const d2 *>(yr + 32 * p + 2 * m);
+            st.y[s][2 * p] = ok ? v.x : 0.0;
+            st.y[s][2 * p + 1] = ok ? v.y : 0.0;
+        }
+        if (ND & 1) {
+            const double v = yr[16 * (ND - 1) + m];
+            st.y[s][ND - 1] = ok ? v : 0.0;
         }
     }
 }
 
-template <int RB, int ND>
-__device__ __forceinline__ void adj_fetch_step(AdjStep<RB, ND> &st, const AdjMfmaArgs &a,
-                                               const float *const rowp[RB], int j, int j_end, int m, int q) {
-    if (j + kAdjMfmaStep <= j_end)
-        adj_fetch<RB, ND, true>(st, a, rowp, j, j_end, m, q);
-    else
-        adj_fetch<RB, ND, false>(st, a, rowp, j, j_end, m, q);
+// cost entries of one step: columns col0 + s (s < 4) of this lane's rows
+template <int RB, bool FULL>
+__device__ __forceinline__ void adj_fetch_cost(float c[RB][kAdjMfmaE], const AdjMfmaArgs &a,
+                                               const float *const rowp[RB], int col0, int j_end) {
+    constexpr int E = kAdjMfmaE;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        if (FULL && a.vec) {
+            const f4 v = *reinterpret_cast<const f4 *>(rowp[rb] + col0);
+            c[rb][0] = v.x;
+            c[rb][1] = v.y;
+            c[rb][2] = v.z;
+            c[rb][3] = v.w;
+        } else {
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                const int col = FULL ? col0 + s : min(col0 + s, j_end - 1);
+                const float x = rowp[rb][col];
+                c[rb][s] = FULL || col0 + s < j_end ? x : 0.f;
+            }
+        }
+    }
 }
 
 // Entries whose row and column factors lie in [2^-500, 2^500] and whose cost
@@ -240,22 +262,22 @@ __device__ __forceinline__ void adj_fetch_step(AdjStep<RB, ND> &st, const AdjMfm
 // and MFMAs form one basic block per step, so the matrix-core products of one
 // entry overlap the exponentials of the next.  The others (none in most
 // sweeps) enter that pass with weight 0 and are added afterwards with R formed
-// directly, under one branch per step.
+// directly, under one branch per step.  Columns col0 + s, s < 4.
 template <int RB, int ND>
-__device__ __forceinline__ void adj_compute(const AdjStep<RB, ND> &st, const AdjMfmaArgs &a, const bool row_ok[RB],
-                                            const bool row_in[RB], const double fr[RB], const double ar[RB],
-                                            float c_lim, int j, int j_end, int q, d4 acc[RB][ND],
-                                            double acc_r[RB], double acc_w[RB], double &swc) {
+__device__ __forceinline__ void adj_compute(const float cv[RB][kAdjMfmaE], const AdjYStep<ND> &st,
+                                            const AdjMfmaArgs &a, const bool row_ok[RB], const bool row_in[RB],
+                                            const double fr[RB], const double ar[RB], int col0, int j_end,
+                                            d4 acc[RB][ND], double acc_r[RB], double acc_w[RB], double &swc) {
     constexpr int E = kAdjMfmaE;
-    const int cb = j + E * q;
+    const float c_lim = a.c_lim;
     bool any_slow = false;
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-        const bool colok = cb + s < j_end;
+        const bool colok = col0 + s < j_end;
         const bool col_in = st.b[s] >= 0x1p-500 && st.b[s] <= 0x1p500;
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) {
-            const float cf = st.c[rb][s];
+            const float cf = cv[rb][s];
             const bool fast = row_ok[rb] && colok && row_in[rb] && col_in && fabsf(cf) <= c_lim;
             any_slow |= row_ok[rb] && colok && !fast;
             const double c = (double)cf;
@@ -273,16 +295,16 @@ __device__ __forceinline__ void adj_compute(const AdjStep<RB, ND> &st, const Adj
     if (__builtin_amdgcn_read_exec() & __ballot(any_slow)) {
 #pragma unroll
         for (int s = 0; s < E; ++s) {
-            const bool colok = cb + s < j_end;
+            const bool colok = col0 + s < j_end;
             const bool col_in = st.b[s] >= 0x1p-500 && st.b[s] <= 0x1p500;
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) {
-                const float cf = st.c[rb][s];
+                const float cf = cv[rb][s];
                 const bool slow = row_ok[rb] && colok && !(row_in[rb] && col_in && fabsf(cf) <= c_lim);
                 double R = 0.0, w = 0.0;
                 const double c = (double)cf;
                 if (slow) {
-                    R = exp((fr[rb] + a.g[cb + s] - c) * a.inv_eps);
+                    R = exp((fr[rb] + a.g[col0 + s] - c) * a.inv_eps);
                     w = (R - exp(-c * a.inv_eps)) * a.inv_ij;
                 }
                 acc_r[rb] += R;
@@ -296,6 +318,58 @@ __device__ __forceinline__ void adj_compute(const AdjStep<RB, ND> &st, const Adj
     }
 }
 
+// The wave's state over one split
+template <int RB, int ND>
+struct AdjWave {
+    const float *rowp[RB];
+    bool row_ok[RB], row_in[RB];
+    double fr[RB], ar[RB], acc_r[RB], acc_w[RB];
+    d4 acc[RB][ND];
+    double swc;
+};
+
+// One super-step of 16 T columns from J0: lane q takes columns J0 + 4 T q +
+// 4 t + s in step t (s < 4), so each lane's 16 T bytes of a row, and the
+// wave's 64 T bytes per row, are read by one set of loads (issued a super-step
+// ahead into `nc`).  Y operands go one step ahead through ya / yb (T even):
+// step 0's are in ya on entry, and the next super-step's step 0 is in ya on
+// exit when `more`.
+template <int RB, int ND>
+__device__ __forceinline__ void adj_super_step(AdjWave<RB, ND> &wv, const AdjMfmaArgs &a,
+                                               const f4 cc[RB][kAdjSuperT], f4 nc[RB][kAdjSuperT],
+                                               AdjYStep<ND> &ya, AdjYStep<ND> &yb, int J0, bool more, int j_end,
+                                               int m, int q) {
+    constexpr int T = kAdjSuperT, W = 16 * kAdjSuperT;
+    static_assert(T % 2 == 0, "Y buffers alternate per step");
+    if (more) {
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+                nc[rb][t] = *reinterpret_cast<const f4 *>(wv.rowp[rb] + J0 + W + 4 * T * q + 4 * t);
+    }
+    const int c0 = J0 + 4 * T * q;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        AdjYStep<ND> &yc = (t & 1) ? yb : ya;
+        AdjYStep<ND> &yn = (t & 1) ? ya : yb;
+        if (t + 1 < T)
+            adj_fetch_y<ND, true>(yn, a, c0 + 4 * (t + 1), j_end, m);
+        else if (more)
+            adj_fetch_y<ND, true>(yn, a, J0 + W + 4 * T * q, j_end, m);
+        float cv[RB][kAdjMfmaE];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            cv[rb][0] = cc[rb][t].x;
+            cv[rb][1] = cc[rb][t].y;
+            cv[rb][2] = cc[rb][t].z;
+            cv[rb][3] = cc[rb][t].w;
+        }
+        adj_compute<RB, ND>(cv, yc, a, wv.row_ok, wv.row_in, wv.fr, wv.ar, c0 + 4 * t, j_end, wv.acc, wv.acc_r,
+                            wv.acc_w, wv.swc);
+    }
+}
+
 template <int RB, int ND>
 __global__ __launch_bounds__(kAdjMfmaWaves * 64) void adjoint_mfma_kernel(AdjMfmaArgs a) {
     const int lane = threadIdx.x & 63;
@@ -306,38 +380,73 @@ __global__ __launch_bounds__(kAdjMfmaWaves * 64) void adjoint_mfma_kernel(AdjMfm
     const int i0 = tile * 16 * RB;
     const int j_begin = split * a.split_cols;
     const int j_end = min(a.n_j, j_begin + a.split_cols);
-    const float *rowp[RB];
-    bool row_ok[RB], row_in[RB];
-    double fr[RB], ar[RB], acc_r[RB], acc_w[RB];
-    d4 acc[RB][ND];
+    AdjWave<RB, ND> wv;
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
         const int r = i0 + 16 * rb + m;
-        row_ok[rb] = r < a.n_i;
-        rowp[rb] = a.A + (int64_t)(row_ok[rb] ? r : 0) * a.lda;
-        fr[rb] = row_ok[rb] ? a.f[r] : 0.0;
-        ar[rb] = exp(fr[rb] * a.inv_eps);
-        row_in[rb] = ar[rb] >= 0x1p-500 && ar[rb] <= 0x1p500;
-        acc_r[rb] = 0.0;
-        acc_w[rb] = 0.0;
+        wv.row_ok[rb] = r < a.n_i;
+        wv.rowp[rb] = a.A + (int64_t)(wv.row_ok[rb] ? r : 0) * a.lda;
+        wv.fr[rb] = wv.row_ok[rb] ? a.f[r] : 0.0;
+        wv.ar[rb] = exp(wv.fr[rb] * a.inv_eps);
+        wv.row_in[rb] = wv.ar[rb] >= 0x1p-500 && wv.ar[rb] <= 0x1p500;
+        wv.acc_r[rb] = 0.0;
+        wv.acc_w[rb] = 0.0;
 #pragma unroll
-        for (int nd = 0; nd < ND; ++nd) acc[rb][nd] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int nd = 0; nd < ND; ++nd) wv.acc[rb][nd] = d4{0.0, 0.0, 0.0, 0.0};
     }
-    double swc = 0.0;
-    AdjStep<RB, ND> sa, sb;
+    wv.swc = 0.0;
     int j = j_begin;
-    if (j < j_end) adj_fetch_step<RB, ND>(sa, a, rowp, j, j_end, m, q);
-    while (j < j_end) {  // two steps per trip: sa and sb alternate, no register copies
-        const bool has1 = j + kAdjMfmaStep < j_end;
-        if (has1) adj_fetch_step<RB, ND>(sb, a, rowp, j + kAdjMfmaStep, j_end, m, q);
-        adj_compute<RB, ND>(sa, a, row_ok, row_in, fr, ar, a.c_lim, j, j_end, q, acc, acc_r, acc_w, swc);
-        if (!has1) break;
-        const bool has2 = j + 2 * kAdjMfmaStep < j_end;
-        if (has2) adj_fetch_step<RB, ND>(sa, a, rowp, j + 2 * kAdjMfmaStep, j_end, m, q);
-        adj_compute<RB, ND>(sb, a, row_ok, row_in, fr, ar, a.c_lim, j + kAdjMfmaStep, j_end, q, acc, acc_r, acc_w, swc);
-        j += 2 * kAdjMfmaStep;
+    // full super-steps of 16 T columns (16-byte cost loads only); two per trip
+    // so the cost buffers alternate without register copies
+    constexpr int SW = 16 * kAdjSuperT;
+    const int n_super = a.vec ? (j_end - j_begin) / SW : 0;
+    if (n_super > 0) {
+        f4 ca[RB][kAdjSuperT], cb[RB][kAdjSuperT];
+        AdjYStep<ND> ya, yb;
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int t = 0; t < kAdjSuperT; ++t)
+                ca[rb][t] = *reinterpret_cast<const f4 *>(wv.rowp[rb] + j + 4 * kAdjSuperT * q + 4 * t);
+        adj_fetch_y<ND, true>(ya, a, j + 4 * kAdjSuperT * q, j_end, m);
+        int S = 0;
+        for (; S + 2 <= n_super; S += 2) {
+            adj_super_step<RB, ND>(wv, a, ca, cb, ya, yb, j_begin + SW * S, true, j_end, m, q);
+            adj_super_step<RB, ND>(wv, a, cb, ca, ya, yb, j_begin + SW * (S + 1), S + 2 < n_super, j_end, m, q);
+        }
+        if (S < n_super) adj_super_step<RB, ND>(wv, a, ca, cb, ya, yb, j_begin + SW * S, false, j_end, m, q);
+        j = j_begin + SW * n_super;
     }
-    // D[i][n] of a 16 x 16 tile sits in lane n + 16 (i % 4) as acc[i / 4]: row q + 4 r, column m
+    // the rest (and everything without 16-byte cost loads): 16-column steps,
+    // lane q taking columns j + 4 q + s, the next step's loads in flight
+    if (j < j_end) {
+        float ca[RB][kAdjMfmaE], cb[RB][kAdjMfmaE];
+        AdjYStep<ND> ya, yb;
+        auto fetch = [&](float (&c)[RB][kAdjMfmaE], AdjYStep<ND> &ys, int jj) {
+            if (jj + kAdjMfmaStep <= j_end) {
+                adj_fetch_cost<RB, true>(c, a, wv.rowp, jj + kAdjMfmaE * q, j_end);
+                adj_fetch_y<ND, true>(ys, a, jj + kAdjMfmaE * q, j_end, m);
+            } else {
+                adj_fetch_cost<RB, false>(c, a, wv.rowp, jj + kAdjMfmaE * q, j_end);
+                adj_fetch_y<ND, false>(ys, a, jj + kAdjMfmaE * q, j_end, m);
+            }
+        };
+        fetch(ca, ya, j);
+        while (j < j_end) {
+            const bool has1 = j + kAdjMfmaStep < j_end;
+            if (has1) fetch(cb, yb, j + kAdjMfmaStep);
+            adj_compute<RB, ND>(ca, ya, a, wv.row_ok, wv.row_in, wv.fr, wv.ar, j + kAdjMfmaE * q, j_end, wv.acc,
+                                wv.acc_r, wv.acc_w, wv.swc);
+            if (!has1) break;
+            const bool has2 = j + 2 * kAdjMfmaStep < j_end;
+            if (has2) fetch(ca, ya, j + 2 * kAdjMfmaStep);
+            adj_compute<RB, ND>(cb, yb, a, wv.row_ok, wv.row_in, wv.fr, wv.ar, j + kAdjMfmaStep + kAdjMfmaE * q,
+                                j_end, wv.acc, wv.acc_r, wv.acc_w, wv.swc);
+            j += 2 * kAdjMfmaStep;
+        }
+    }
+    // D[i][n] of a 16 x 16 tile sits in lane n + 16 (i % 4) as acc[i / 4]: row
+    // q + 4 r, tile column m (coordinate adj_coord(nd, m))
     double *Vp = a.Vp + (int64_t)split * a.n_i * a.d;
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
@@ -345,12 +454,12 @@ __global__ __launch_bounds__(kAdjMfmaWaves * 64) void adjoint_mfma_kernel(AdjMfm
         for (int nd = 0; nd < ND; ++nd)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = i0 + 16 * rb + q + 4 * r, dd = 16 * nd + m;
-                if (row < a.n_i && dd < a.d) Vp[(int64_t)row * a.d + dd] = acc[rb][nd][r];
+                const int row = i0 + 16 * rb + q + 4 * r, dd = adj_coord<ND>(nd, m);
+                if (row < a.n_i && dd < a.d) Vp[(int64_t)row * a.d + dd] = wv.acc[rb][nd][r];
             }
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
-        double x = acc_r[rb], y = acc_w[rb];
+        double x = wv.acc_r[rb], y = wv.acc_w[rb];
         x += __shfl_xor(x, 16);
         y += __shfl_xor(y, 16);
         x += __shfl_xor(x, 32);
@@ -361,9 +470,20 @@ __global__ __launch_bounds__(kAdjMfmaWaves * 64) void adjoint_mfma_kernel(AdjMfm
             a.swp[(int64_t)split * a.n_i + row] = y;
         }
     }
+    double swc = wv.swc;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) swc += __shfl_xor(swc, off);
     if (lane == 0) a.wcp[(int64_t)split * a.n_tiles + tile] = swc;
+}
+
+// Y (J x d) -> J x dp, zero-padded
+__global__ __launch_bounds__(256) void adjoint_ypad_kernel(const double *__restrict__ Y, int n_j, int d, int dp,
+                                                           double *__restrict__ Yp) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n_j * dp) return;
+    const int64_t j = e / dp;
+    const int k = (int)(e % dp);
+    Yp[e] = k < d ? Y[j * d + k] : 0.0;
 }
 
 // exp(g_j / eps) per column
@@ -483,7 +603,7 @@ struct AdjWork {
     // matrix-core path (d <= 64): per-split partials and the column factors
     bool mfma;
     int rb, nd, n_tiles, n_split, split_cols;
-    double *Vp, *rp, *swp, *bcol;
+    double *Vp, *rp, *swp, *bcol, *Yp;
 };
 
 // matrix-core path: rows per wave, splits and columns per split of an I x J x d
@@ -508,12 +628,13 @@ static size_t adj_carve(Arena &ar, int64_t I, int64_t J, int64_t d, AdjWork *w) 
     w->nparts = w->mfma ? w->n_split * w->n_tiles : (int)((I + kAdjRows - 1) / kAdjRows);
     w->partials = ar.take<double>(w->nparts);
     w->scal = ar.take<double>(8);
-    w->Vp = w->rp = w->swp = w->bcol = nullptr;
+    w->Vp = w->rp = w->swp = w->bcol = w->Yp = nullptr;
     if (w->mfma) {
         w->Vp = ar.take<double>((size_t)w->n_split * I * d);
         w->rp = ar.take<double>((size_t)w->n_split * I);
         w->swp = ar.take<double>((size_t)w->n_split * I);
         w->bcol = ar.take<double>(J);
+        w->Yp = ar.take<double>((size_t)J * 16 * w->nd);
     }
     return ar.off;
 }
@@ -556,7 +677,9 @@ extern "C" int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int
         WOTK_LAUNCH_CHECK();
         // |C| <= c_lim keeps C/eps within 690 < 1000 ln 2 (FP32 rounding toward zero)
         const float c_lim = std::nextafter((float)(690.0 * epsilon), 0.f);
-        AdjMfmaArgs a{C, ldc, (int)I, (int)J, (int)d, f, g, w.bcol, Y, inv_eps, inv_ij, c_lim, w.split_cols, w.n_tiles,
+        adjoint_ypad_kernel<<<(unsigned)((J * 16 * w.nd + 255) / 256), 256, 0, s>>>(Y, (int)J, (int)d, 16 * w.nd, w.Yp);
+        WOTK_LAUNCH_CHECK();
+        AdjMfmaArgs a{C, ldc, (int)I, (int)J, (int)d, f, g, w.bcol, w.Yp, inv_eps, inv_ij, c_lim, w.split_cols, w.n_tiles,
                       ldc % 4 == 0 && reinterpret_cast<uintptr_t>(C) % 16 == 0, w.Vp, w.rp, w.swp, w.partials};
         switch (w.nd) {
             case 1: launch_adj_mfma<1>(a, w.n_split, s); break;
