@@ -222,7 +222,11 @@ def valu_rates(torch):
     rates = (ctypes.c_double * 7)()
     _native.check(L.wotk_probe_valu_rates(rates, 7, dev._ptr(sink), sink.numel() * 4, dev._stream(torch)))
     names = ['exp_f32', 'fma_f64', 'fma_f32', 'add_f64_cvt_pair', 'rndne_f64_add', 'ldexp_f32', 'split_exp_entry']
-    return {n: float(v) for n, v in zip(names, rates)}
+    out = {n: float(v) for n, v in zip(names, rates)}
+    mix = (ctypes.c_double * 4)()
+    _native.check(L.wotk_probe_issue_mix(mix, 4, dev._ptr(sink), sink.numel() * 4, dev._stream(torch)))
+    out['fma29_exp1_mix_by_waves_per_simd'] = {str(w): float(v) for w, v in zip((1, 2, 4, 8), mix)}
+    return out
 
 
 def bench_timecourse(args):

@@ -472,6 +472,10 @@ int wotk_get_tile(int32_t kernel, int32_t bucket, wotk_tile *out);
  * ------------------------------------------------------------------------- */
 size_t wotk_probe_sink_bytes(void);
 int wotk_probe_valu_rates(double *rates_host, int32_t n_rates, void *sink, size_t sink_bytes, void *stream);
+/* Issue rate (lane-operations per second over the device) of an FP32 mix of
+ * 29 FMAs and one v_exp_f32 per step, about the batched sweep's ratio, at 1,
+ * 2, 4 and 8 waves per SIMD: rates_host (HOST, >= 4 entries).  Same sink. */
+int wotk_probe_issue_mix(double *rates_host, int32_t n_rates, void *sink, size_t sink_bytes, void *stream);
 
 #ifdef __cplusplus
 }

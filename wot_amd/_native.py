@@ -129,6 +129,7 @@ SIGNATURES = {
     'wotk_pca_center': (ctypes.c_int, [_P, _I32, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _SZ, _P]),
     'wotk_probe_sink_bytes': (_SZ, []),
     'wotk_probe_valu_rates': (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), _I32, _P, _SZ, _P]),
+    'wotk_probe_issue_mix': (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), _I32, _P, _SZ, _P]),
     'wotk_tile_variants': (_I32, [_I32, ctypes.POINTER(Tile), _I32]),
     'wotk_shape_bucket': (_I32, [_I32, _I64, _I64, _I32]),
     'wotk_num_buckets': (_I32, [_I32]),

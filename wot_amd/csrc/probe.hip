@@ -97,9 +97,70 @@ static int probe_time(float *sink, int grid, hipStream_t s, double *ops_per_s) {
     return WOTK_OK;
 }
 
+// Issue rate of an FMA / exponential mix (29 dependent-chain FP32 FMAs and one
+// v_exp_f32 per step and chain, about the batched sweep's ratio) at a given
+// number of waves per SIMD: 4-wave workgroups, one per CU per wave level.
+constexpr int kMixFmas = 29;
+
+__global__ __launch_bounds__(256) void probe_mix_kernel(float *sink, float seed, int steps) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    float x[kProbeChains];
+#pragma unroll
+    for (int c = 0; c < kProbeChains; ++c) x[c] = seed + 0.001f * (float)(c + (tid & 7));
+    for (int s = 0; s < steps; ++s) {
+#pragma unroll
+        for (int c = 0; c < kProbeChains; ++c) {
+#pragma unroll
+            for (int u = 0; u < kMixFmas; ++u) x[c] = fmaf(x[c], 0.999f, 1.0e-3f);
+            x[c] = probe_exp2(-x[c]);  // stays in (0.5, 1]
+        }
+    }
+    float acc = 0.f;
+#pragma unroll
+    for (int c = 0; c < kProbeChains; ++c) acc += x[c];
+    if (acc == -1.0f) sink[tid] = acc;  // never true for these chains
+}
+
+constexpr int kMixWaveLevels[] = {1, 2, 4, 8};
+constexpr int kMixLevels = 4;
+
 }  // namespace wotk
 
 using namespace wotk;
+
+extern "C" int wotk_probe_issue_mix(double *rates_host, int32_t n_rates, void *sink, size_t sink_bytes,
+                                    void *stream) {
+    WOTK_REQUIRE(rates_host != nullptr && n_rates >= kMixLevels, "rates_host needs %d entries", kMixLevels);
+    hipStream_t s = (hipStream_t)stream;
+    int dev = 0, ncu = 256;
+    WOTK_HIP(hipGetDevice(&dev));
+    WOTK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    WOTK_REQUIRE(sink != nullptr && sink_bytes >= (size_t)ncu * 8 * 256 * sizeof(float), "sink needs %zu bytes",
+                 (size_t)ncu * 8 * 256 * sizeof(float));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    for (int lv = 0; lv < kMixLevels && e == hipSuccess; ++lv) {
+        const int w = kMixWaveLevels[lv], grid = ncu * w;
+        const int steps = 256 * 8 / w;  // about the same duration at every level
+        probe_mix_kernel<<<grid, 256, 0, s>>>((float *)sink, 0.25f, steps);  // warm-up
+        e = hipEventRecord(ev[0], s);
+        if (e == hipSuccess) {
+            probe_mix_kernel<<<grid, 256, 0, s>>>((float *)sink, 0.25f, steps);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (e == hipSuccess)
+            rates_host[lv] = (double)grid * 256.0 * kProbeChains * steps * (kMixFmas + 1) / (ms * 1e-3);
+    }
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e != hipSuccess) return fail(WOTK_EHIP, "issue-mix probe failed: %s", hipGetErrorString(e));
+    return WOTK_OK;
+}
 
 extern "C" int wotk_probe_valu_rates(double *rates_host, int32_t n_rates, void *sink, size_t sink_bytes,
                                      void *stream) {
