@@ -79,6 +79,14 @@ def test_new_entry_points_reject_bad_arguments_on_the_host():
     assert lib.wotk_interp_weights(8, 2, 4, 3, 0.5, 8, 8, None) == _native.WOTK_EINVAL  # ld < J
     assert lib.wotk_interp_gather(8, 1, 8, 1, 4, 8, 3, 3, 3, 0.5, 8, 3, None) == _native.WOTK_EINVAL  # ld0 < d
     assert lib.wotk_rows_normalize_f64(8, 3, 4, 2, None) == _native.WOTK_EINVAL  # ld < k
+    few = (ctypes.c_double * 2)()
+    assert lib.wotk_probe_issue_mix(few, 2, 8, 1 << 20, None) == _native.WOTK_EINVAL  # needs 4 rates
+    assert lib.wotk_probe_valu_rates(few, 2, 8, 1 << 20, None) == _native.WOTK_EINVAL  # needs 7 rates
+    # adjoint: shape checks before any device work; workspace grows with the column splits
+    assert lib.wotk_ot_adjoint(8, 0, 8, 0, 4, 4, 8, 8, 2, 8, 1.0, 8, 8, 1.0, 50.0, 0.05, 8, None, None, None, 8,
+                               1 << 20, None) == _native.WOTK_EINVAL  # ldc < J
+    # 100 x 5000, d = 30: 20 column splits of 256 columns, each with its own 100 x 30 FP64 partial V
+    assert lib.wotk_adjoint_workspace_size(100, 5000, 30) >= 20 * 100 * 30 * 8
 
 
 def test_emd_is_host_code():
