@@ -141,19 +141,23 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_rows_kernel(
 // The same rows pass on the matrix cores (d <= 64), split over columns.
 //
 // A wave owns RB blocks of 16 rows and the columns of one split; lane
-// (m, q) = (lane % 16, lane / 16) forms W for row 16 rb + m and columns
-// j + 4 q + s (s < 4) of each 16-column step: W is the A operand (16 x 4,
-// k = q) of four k-steps of v_mfma_f64_16x16x4f64 and Y of the same columns
-// (coordinates 16 nd + m) the B operand, so V = W Y runs on the matrix cores
-// beside the exponentials.  Every lane reads 16 contiguous bytes of each of
-// its rows per step; the next step's cost, column factors and Y are loaded
-// while the current one computes.  No LDS and no barriers.
+// (m, q) = (lane % 16, lane / 16) forms W for row 16 rb + m and four columns
+// chosen by q in each 16-column step: W is the A operand (16 x 4, k = q) of
+// four k-steps of v_mfma_f64_16x16x4f64 and Y of the same columns (Y
+// zero-padded, coordinates laid on the tiles by adj_coord) the B operand, so
+// V = W Y runs on the matrix cores beside the exponentials.  Any assignment
+// of columns to (step, q) works as long as W and Y follow the same one: in
+// super-steps of 2 steps lane q takes the 8 contiguous columns J0 + 8 q ...
+// (32 bytes of each of its rows, the wave a full line per row at once); the
+// rest of a split goes in single steps (columns j + 4 q + s).  The next
+// super-step's cost and the next step's column factors and Y are loaded while
+// the current one computes.  No LDS and no barriers.
 //
 // One exponential per entry: R = exp(-C/eps) exp(f_i/eps) exp(g_j/eps), the
 // row's factor formed once per lane and the column's by
-// adjoint_colexp_kernel, when exp(-C/eps) and the factor product both lie in
-// [2^-1000, 2^1000]; otherwise (an underflowing K~, an overflowing factor,
-// NaN) R = exp((f_i + g_j - C)/eps) directly.
+// adjoint_colexp_kernel, when both factors lie in [2^-500, 2^500] and
+// |C| <= 690 eps; otherwise (an underflowing K~, an overflowing factor, NaN)
+// R = exp((f_i + g_j - C)/eps) directly.
 //
 // Outputs per split (summed over splits in a fixed order afterwards):
 // V (n_i x d), the row sums of R and W, and per wave sum W C.
