@@ -49,6 +49,30 @@ def test_batch_grid_matches_oracle(batch, share_exp):
     print('batch worst rel frob %.3e (share_exp=%s, exact rows %d)' % (worst, share_exp, res.stats['exact_rows']))
 
 
+@pytest.mark.parametrize('inner_iter_max', [50, 5])
+def test_batch_tiny_days(batch, inner_iter_max):
+    """Days of 1, 2, 3 and 5 cells beside a 130-cell day: single-row and
+    single-column pairs, widths below one 4-column group and far below a tile,
+    every (eps, lambda1) combination against the oracle.  With eps changing
+    every 5 iterations the schedule converges inside the run, so the shared
+    modes (1, 1), (r, r), (2, r) and (2, 10) all run."""
+    sched = dict(SCHED, inner_iter_max=inner_iter_max)
+    coords, growth = _course(21, [1, 3, 2, 5, 130, 1])
+    res = batch.solve_time_course(coords, growth, epsilons=(0.01, 0.05, 0.1), lambda1s=(0.5, 1.0, 5.0),
+                                  graph_iters=50, **sched)
+    worst = 0.0
+    for p in range(len(coords) - 1):
+        C = O.default_cost_matrix(coords[p], coords[p + 1])
+        for e, (eps, lam1) in enumerate(res.combos):
+            R = res.plan(p, e)
+            Rref = O.transport_stablev2(C, lam1, 50.0, eps, G=growth[p], epsilon0=1.0, **sched)
+            err = np.linalg.norm(R - Rref) / np.linalg.norm(Rref)
+            marg = max(np.abs(R.sum(a) - Rref.sum(a)).max() / np.abs(Rref.sum(a)).max() for a in (0, 1))
+            worst = max(worst, err)
+            assert err <= 1e-5 and marg <= 1e-6, (p, C.shape, eps, lam1, err, marg)
+    print('tiny days (inner_iter_max %d): worst rel frob %.3e' % (inner_iter_max, worst))
+
+
 @pytest.mark.parametrize('epsilons,lambda1s,group', [
     ((0.05, 0.1), (1.0, 5.0), 2),          # 2 groups of 2
     ((0.05, 0.1), (0.5, 1.0, 5.0), 3),     # 2 groups of 3

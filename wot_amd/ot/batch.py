@@ -146,10 +146,10 @@ def solve_time_course(coords, growth, epsilons=(0.05,), lambda1s=(1.0,), lambda2
     (each cost entry is then read once per group instead of once overall).
     ``share_exp`` lets combinations of equal epsilon share one exponential per
     cost entry (libwotk ``WOTK_BATCH_SHARE_EXP``); ``share_eps`` also lets
-    three eps groups share the last group's FP64 exponent: powers of its
-    exponential where k = log2(e)/eps stands in an integer ratio (the
-    converged schedule), the real multiple of its exponent elsewhere
-    (``WOTK_BATCH_SHARE_EPS``)."""
+    three eps groups share the last group's exponent: powers of its
+    exponential (exponent split exactly in FP32) where k = log2(e)/eps stands
+    in an integer ratio (the converged schedule), the real multiple of its
+    FP64 exponent elsewhere (``WOTK_BATCH_SHARE_EPS``)."""
     tc = prepare_time_course(coords, growth, pairs=pairs, cost_mode=cost_mode, device=device)
     return solve_prepared(tc, epsilons, lambda1s, lambda2=lambda2, scaling_iter=scaling_iter, tau=tau,
                           epsilon0=epsilon0, extra_iter=extra_iter, inner_iter_max=inner_iter_max,
