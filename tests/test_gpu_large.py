@@ -112,6 +112,18 @@ def test_fused_vs_oracle(mods, n_i, n_j, dim, seed, marg_tol):
     assert err <= 1e-5 and marg <= marg_tol
 
 
+@pytest.mark.parametrize('n_i,n_j,dim', [(1, 5, 3), (5, 1, 3), (3, 2, 7), (33, 17, 30), (40, 200, 50)])
+def test_fused_small_shapes_vs_oracle(mods, n_i, n_j, dim):
+    """Pairs below one 32-row MFMA tile and one 32-column tile (single rows and
+    columns, clamped indices and masked entries) against the FP64 oracle."""
+    torch, ot, dev, large = mods
+    from wot_amd.synthetic import day_pair
+    dp = day_pair(n_i, n_j, dim=dim, n_components=3, seed=17)
+    err, marg = _vs_oracle(torch, large, dp.x, dp.y, dp.growth, 6, SCHED)
+    print('fused vs oracle %dx%d d=%d: rel frob %.3e marginals %.3e' % (n_i, n_j, dim, err, marg))
+    assert err <= 1e-5 and marg <= 1e-6, (err, marg)
+
+
 def test_fused_outlier_vs_oracle(mods):
     """eps0 = eps = 0.01 with an outlier source cell (its row is recomputed by
     the repair kernel), against the FP64 oracle."""
