@@ -86,6 +86,8 @@ def test_adjoint_kernel_vs_fp64_autograd(mods, n_i, n_j, d, solver):
     (100, 5000, 30, 0.05, False),  # 20 column splits, the last one ragged
     (200, 250, 70, 0.05, False),   # d > 64: the LDS kernel
     (300, 400, 30, 0.002, False),  # exp(-C/eps) below 2^-1000 for ~9% of the entries: R formed directly
+    (1, 3, 5, 0.05, False),        # one row, three columns: a single masked step
+    (5, 1, 30, 0.05, False),       # one column
 ])
 def test_adjoint_shapes_against_autograd(mods, n_i, n_j, d, eps, odd_ld):
     """Adjoint at arbitrary potentials (the partial derivative at a fixed plan
