@@ -707,23 +707,24 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
         float bv[NBS][PER];
 #define WOTK_LDS_FETCH(k)                                                                        \
     _Pragma("unroll") for (int p = 0; p < PER; ++p) {                                           \
-        const int j_ = tid + p * THREADS;                                                       \
-        const int col_ = (k) * CH + j_;                                                         \
-        const bool ok_ = j_ < CH && col_ < ncols4;                                              \
+        /* the column is clamped into the padded row, so every load is in bounds and none */    \
+        /* sits under a lane mask; staged columns at or past ncols4 are never read (a lane */   \
+        /* computes its 4 columns only below ncols4), and threads past CH commit nothing */     \
+        const int cc_ = min((k) * CH + tid + p * THREADS, ncols4 - 1);                          \
         _Pragma("unroll") for (int g = 0; g < NG; ++g) {                                        \
             if (staged(g)) {                                                                    \
-                const long long at_ = (long long)(g * NL + L0) * pd.other_ld + col_;            \
-                psv[xslot(g)][p] = ok_ ? PS[at_] : 0.f;                                         \
-                pmv[xslot(g)][p] = ok_ ? PM[at_] : -(float)kShiftClamp;                         \
+                const long long at_ = (long long)(g * NL + L0) * pd.other_ld + cc_;             \
+                psv[xslot(g)][p] = PS[at_];                                                     \
+                pmv[xslot(g)][p] = PM[at_];                                                     \
             }                                                                                   \
             if (pw(g) == 0) {                                                                   \
                 _Pragma("unroll") for (int i = 0; i < NL - 1; ++i) {                            \
                     const int l_ = i < L0 ? i : i + 1;                                          \
-                    bv[bslot(g) + i][p] = ok_ ? B[(long long)(g * NL + l_) * pd.other_ld + col_] : 0.f; \
+                    bv[bslot(g) + i][p] = B[(long long)(g * NL + l_) * pd.other_ld + cc_];      \
                 }                                                                               \
             } else {                                                                            \
                 _Pragma("unroll") for (int l = 0; l < NL; ++l)                                  \
-                    bv[bslot(g) + l][p] = ok_ ? B[(long long)(g * NL + l) * pd.other_ld + col_] : 0.f; \
+                    bv[bslot(g) + l][p] = B[(long long)(g * NL + l) * pd.other_ld + cc_];       \
             }                                                                                   \
         }                                                                                       \
     }
@@ -741,23 +742,27 @@ __global__ __launch_bounds__(NWAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 
         WOTK_LDS_FETCH(0)
         WOTK_LDS_COMMIT(0)
         static_assert(CH == 256, "prefetch assumes one 256-column step per chunk");
+        // lanes past the row's (padded) end re-read its last 16 bytes: every load
+        // is in bounds and needs no lane mask, and those entries are never used
         float4 cnext[WROWS];
 #pragma unroll
-        for (int r = 0; r < WROWS; ++r) cnext[r] = lane * 4 < ncols4 ? bload4(rowp[r] + lane * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int r = 0; r < WROWS; ++r) cnext[r] = bload4(rowp[r] + min(lane * 4, ncols4 - 4));
         __syncthreads();
         for (int k = 0; k < nchunks; ++k) {
             const int buf = k % NBUF;
-            if (k + 1 < nchunks) {  // in flight while this chunk computes
-                WOTK_LDS_FETCH(k + 1)
-            }
             const int cl = lane * 4;  // column within the chunk
             const int c = k * CH + cl;
+            // this chunk's cost entries leave the prefetch registers before the next
+            // chunk's loads (cost and staging) are issued: nothing below waits on them
+            // until the commit
             float4 ccur[WROWS];
 #pragma unroll
             for (int r = 0; r < WROWS; ++r) ccur[r] = cnext[r];
-            if (k + 1 < nchunks && (k + 1) * CH + cl < ncols4) {
+            if (k + 1 < nchunks) {  // wave-uniform; in flight while this chunk computes
+                const int cn = min((k + 1) * CH + cl, ncols4 - 4);
 #pragma unroll
-                for (int r = 0; r < WROWS; ++r) cnext[r] = bload4(rowp[r] + (k + 1) * CH + cl);
+                for (int r = 0; r < WROWS; ++r) cnext[r] = bload4(rowp[r] + cn);
+                WOTK_LDS_FETCH(k + 1)
             }
             if (c < ncols4) {
                 // the base group first: the others take powers of its terms, or
