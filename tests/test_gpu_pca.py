@@ -29,7 +29,8 @@ def rel(a, b):
 
 @pytest.mark.parametrize('shape', [(1, 1, 3), (7, 5, 300), (130, 70, 97), (300, 257, 1031),
                                    (300, 257, 1100), (40, 33, 2052),  # 16-byte rows: ragged chunks and row blocks
-                                   (1500, 1200, 260), (1500, 1200, 97)])  # 85 streaming row blocks: two-level column means
+                                   (1500, 1200, 260), (1500, 1200, 97),  # 85 streaming row blocks: two-level column means
+                                   (600000, 500000, 4)])  # 68,750 16-row copy blocks on a 65,535-block grid
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
 def test_center_kernel(pca, shape, dtype):
     import torch

@@ -236,19 +236,21 @@ __global__ __launch_bounds__(256) void center_apply2_kernel(const T *__restrict_
     typedef double d2 __attribute__((ext_vector_type(2)));
     typedef T V2 __attribute__((ext_vector_type(2)));
     const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 2;
-    const int64_t r0 = (int64_t)blockIdx.y * kApplyRows;
     if (j >= g) return;
     const d2 m = *reinterpret_cast<const d2 *>(mu + j);
-    const int rows = (int)min<int64_t>(kApplyRows, n - r0);
+    // row blocks strided over the grid's y extent (at most 65535 blocks)
+    for (int64_t r0 = (int64_t)blockIdx.y * kApplyRows; r0 < n; r0 += (int64_t)gridDim.y * kApplyRows) {
+        const int rows = (int)min<int64_t>(kApplyRows, n - r0);
 #pragma unroll 4
-    for (int r = 0; r < rows; ++r) {
-        const int64_t row = r0 + r;
-        const V2 v = *reinterpret_cast<const V2 *>(x + row * ldx + j);
-        const double c = nu[row];
-        d2 o;
-        o.x = ((double)v.x - m.x) - c;
-        o.y = ((double)v.y - m.y) - c;
-        *reinterpret_cast<d2 *>(A + row * lda + j) = o;
+        for (int r = 0; r < rows; ++r) {
+            const int64_t row = r0 + r;
+            const V2 v = *reinterpret_cast<const V2 *>(x + row * ldx + j);
+            const double c = nu[row];
+            d2 o;
+            o.x = ((double)v.x - m.x) - c;
+            o.y = ((double)v.y - m.y) - c;
+            *reinterpret_cast<d2 *>(A + row * lda + j) = o;
+        }
     }
 }
 
@@ -304,7 +306,8 @@ int center_impl(const T *x, int64_t ldx, int64_t n, int64_t g, double *A, int64_
         const unsigned n_tr = (unsigned)((n + kTileRows - 1) / kTileRows);
         const bool x2 = reinterpret_cast<uintptr_t>(x) % (2 * sizeof(T)) == 0 && ldx % 2 == 0;
         if (x2 && g % 2 == 0 && aligned16(A) && lda % 2 == 0 && aligned16(mu))
-            center_apply2_kernel<T><<<dim3((unsigned)((g / 2 + 255) / 256), (unsigned)((n + kApplyRows - 1) / kApplyRows)),
+            center_apply2_kernel<T><<<dim3((unsigned)((g / 2 + 255) / 256),
+                                           (unsigned)std::min<int64_t>(65535, (n + kApplyRows - 1) / kApplyRows)),
                                       256, 0, st>>>(x, ldx, n, g, mu, nu, A, lda);
         else
             center_apply_kernel<T><<<dim3((unsigned)((g + 255) / 256), n_tr), 256, 0, st>>>(x, ldx, n, g, mu, nu,
@@ -334,7 +337,7 @@ extern "C" int wotk_pca_center(const void *x, int32_t dtype, int64_t ldx, int64_
     WOTK_REQUIRE(dtype == WOTK_DTYPE_F64 || dtype == WOTK_DTYPE_F32, "pca_center: bad dtype %d", dtype);
     WOTK_REQUIRE(gene_mean && cell_mean, "pca_center: gene_mean and cell_mean are required");
     WOTK_REQUIRE(!A || lda >= g, "pca_center: lda %lld < %lld", (long long)lda, (long long)g);
-    WOTK_REQUIRE((n + kApplyRows - 1) / kApplyRows < 65536,
+    WOTK_REQUIRE((n + kTileRows - 1) / kTileRows < 65536,
                  "pca_center: too many cells (%lld)", (long long)n);
     Arena ar(workspace, workspace_bytes);
     CenterLayout L = center_carve(ar, n, g);
