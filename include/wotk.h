@@ -75,6 +75,11 @@ int wotk_abi_version(void);
  * accumulated in FP32 (x = hi + mid + lo in BF16; 3 terms hi.hi + hi.mid +
  * mid.hi, 6 terms add hi.lo + lo.hi + mid.mid).  The median is exact (radix select over the FP64
  * bit patterns; even count -> mean of the two middle values, numpy.median).
+ * Layout every sweep over C / CT assumes: ldc and ldct multiples of 4, and
+ * columns J .. round_up(J, 4) - 1 of every row of C (I .. round_up(I, 4) - 1
+ * of CT) holding finite values -- the sweeps read whole 16-byte groups.  This
+ * function and wotk_cost_import write those columns as 0; a caller that
+ * builds C / CT itself must do the same.
  * ------------------------------------------------------------------------- */
 size_t wotk_cost_workspace_size(int64_t I, int64_t J, int64_t d, int32_t mode);
 int wotk_cost_sqeuclid(const double *X, const double *Y, int64_t I, int64_t J, int64_t d,
@@ -163,7 +168,8 @@ int wotk_plan(const float *C, int64_t ldc, int64_t I, int64_t J, const double *f
  * (epsilon, lambda1) combinations (the reference sweeps a grid as one process
  * per combination), transport_stablev2 semantics for each solve.  All costs
  * live in one FP32 buffer; pair p's C_p and C_p^T start at c_offset /
- * ct_offset (multiples of 4 elements).  G holds the concatenated growth
+ * ct_offset (multiples of 4 elements), laid out as wotk_cost_sqeuclid writes
+ * them (finite padding columns up to a multiple of 4).  G holds the concatenated growth
  * vectors.  pairs, epsilon and lambda1 are HOST arrays.  f_out / g_out
  * (device, may be NULL) receive the final potentials as consecutive blocks
  * [pair][combination][I_p] (resp. [J_p]).  graph_iters > 0 captures that many
