@@ -181,6 +181,7 @@ struct AdjMfmaArgs {
     int split_cols, n_tiles;
     bool vec;  // 16-byte cost loads (lda and the base 16-byte aligned)
     double *Vp, *rp, *swp, *wcp;  // [split][n_i][d], [split][n_i], [split][n_i], [split][n_tiles]
+    double *cpart;                // [n_tiles][n_j]: column sums of R over each wave's rows
 };
 
 typedef double d2 __attribute__((ext_vector_type(2)));
@@ -271,14 +272,17 @@ template <int RB, int ND>
 __device__ __forceinline__ void adj_compute(const float cv[RB][kAdjMfmaE], const AdjYStep<ND> &st,
                                             const AdjMfmaArgs &a, const bool row_ok[RB], const bool row_in[RB],
                                             const double fr[RB], const double ar[RB], int col0, int j_end,
-                                            d4 acc[RB][ND], double acc_r[RB], double acc_w[RB], double &swc) {
+                                            d4 acc[RB][ND], double acc_r[RB], double acc_w[RB], double &swc,
+                                            double *cprow, int m) {
     constexpr int E = kAdjMfmaE;
     const float c_lim = a.c_lim;
     bool any_slow = false;
+    double csum[E];  // R summed over this lane's rows, per column
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         const bool colok = col0 + s < j_end;
         const bool col_in = st.b[s] >= 0x1p-500 && st.b[s] <= 0x1p500;
+        csum[s] = 0.0;
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) {
             const float cf = cv[rb][s];
@@ -289,6 +293,7 @@ __device__ __forceinline__ void adj_compute(const float cv[RB][kAdjMfmaE], const
             const double R = kt * (ar[rb] * st.b[s]);
             const double w = fast ? (R - kt) * a.inv_ij : 0.0;
             acc_r[rb] += fast ? R : 0.0;
+            csum[s] += fast ? R : 0.0;
             acc_w[rb] += w;
             swc = fma(w, c, swc);
 #pragma unroll
@@ -312,6 +317,7 @@ __device__ __forceinline__ void adj_compute(const float cv[RB][kAdjMfmaE], const
                     w = (R - exp(-c * a.inv_eps)) * a.inv_ij;
                 }
                 acc_r[rb] += R;
+                csum[s] += R;
                 acc_w[rb] += w;
                 swc = fma(w, c, swc);
 #pragma unroll
@@ -319,6 +325,15 @@ __device__ __forceinline__ void adj_compute(const float cv[RB][kAdjMfmaE], const
                     acc[rb][nd] = __builtin_amdgcn_mfma_f64_16x16x4f64(w, st.y[s][nd], acc[rb][nd], 0, 0, 0);
             }
         }
+    }
+    // the column's sum over the wave's rows: the 16 lanes of a q-group share
+    // the columns (fixed xor order); lane m = 0 writes it
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        double v = csum[s];
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off);
+        if (m == 0 && col0 + s < j_end) cprow[col0 + s] = v;
     }
 }
 
@@ -330,6 +345,7 @@ struct AdjWave {
     double fr[RB], ar[RB], acc_r[RB], acc_w[RB];
     d4 acc[RB][ND];
     double swc;
+    double *cprow;  // this row tile's column partials
 };
 
 // One super-step of 16 T columns from J0: lane q takes columns J0 + 4 T q +
@@ -370,7 +386,7 @@ __device__ __forceinline__ void adj_super_step(AdjWave<RB, ND> &wv, const AdjMfm
             cv[rb][3] = cc[rb][t].w;
         }
         adj_compute<RB, ND>(cv, yc, a, wv.row_ok, wv.row_in, wv.fr, wv.ar, c0 + 4 * t, j_end, wv.acc, wv.acc_r,
-                            wv.acc_w, wv.swc);
+                            wv.acc_w, wv.swc, wv.cprow, m);
     }
 }
 
@@ -399,6 +415,7 @@ __global__ __launch_bounds__(kAdjMfmaWaves * 64) void adjoint_mfma_kernel(AdjMfm
         for (int nd = 0; nd < ND; ++nd) wv.acc[rb][nd] = d4{0.0, 0.0, 0.0, 0.0};
     }
     wv.swc = 0.0;
+    wv.cprow = a.cpart + (int64_t)tile * a.n_j;
     int j = j_begin;
     // full super-steps of 16 T columns (16-byte cost loads only); two per trip
     // so the cost buffers alternate without register copies
@@ -440,12 +457,12 @@ __global__ __launch_bounds__(kAdjMfmaWaves * 64) void adjoint_mfma_kernel(AdjMfm
             const bool has1 = j + kAdjMfmaStep < j_end;
             if (has1) fetch(cb, yb, j + kAdjMfmaStep);
             adj_compute<RB, ND>(ca, ya, a, wv.row_ok, wv.row_in, wv.fr, wv.ar, j + kAdjMfmaE * q, j_end, wv.acc,
-                                wv.acc_r, wv.acc_w, wv.swc);
+                                wv.acc_r, wv.acc_w, wv.swc, wv.cprow, m);
             if (!has1) break;
             const bool has2 = j + 2 * kAdjMfmaStep < j_end;
             if (has2) fetch(ca, ya, j + 2 * kAdjMfmaStep);
             adj_compute<RB, ND>(cb, yb, a, wv.row_ok, wv.row_in, wv.fr, wv.ar, j + kAdjMfmaStep + kAdjMfmaE * q,
-                                j_end, wv.acc, wv.acc_r, wv.acc_w, wv.swc);
+                                j_end, wv.acc, wv.acc_r, wv.acc_w, wv.swc, wv.cprow, m);
             j += 2 * kAdjMfmaStep;
         }
     }
@@ -519,6 +536,34 @@ __global__ __launch_bounds__(256) void adjoint_split_sum_kernel(const double *__
         r[e] = x;
         sw[e] = y;
     }
+}
+
+// Column sums of R (the plan's, y_j = c_j / I) from the row tiles' partials
+// in two fixed-order levels: cseg[s][j] over the 64 row tiles of segment s
+// (64 columns x 4 parts per workgroup), then c_j = sum_s cseg[s][j].
+constexpr int kAdjColSeg = 64;
+
+__global__ __launch_bounds__(256) void adjoint_colseg_kernel(const double *__restrict__ cpart, int n_tiles, int n_j,
+                                                             double *__restrict__ cseg) {
+    __shared__ double red[4][64];
+    const int c = threadIdx.x & 63, part = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + c;
+    const int t0 = blockIdx.y * kAdjColSeg, t1 = min(n_tiles, t0 + kAdjColSeg);
+    double x = 0.0;
+    if (j < n_j)
+        for (int t = t0 + part; t < t1; t += 4) x += cpart[(int64_t)t * n_j + j];
+    red[part][c] = x;
+    __syncthreads();
+    if (part == 0 && j < n_j) cseg[(int64_t)blockIdx.y * n_j + j] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+}
+
+__global__ __launch_bounds__(256) void adjoint_colfinal_kernel(const double *__restrict__ cseg, int n_seg, int n_j,
+                                                               double *__restrict__ c) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n_j) return;
+    double x = 0.0;
+    for (int k = 0; k < n_seg; ++k) x += cseg[(int64_t)k * n_j + j];
+    c[j] = x;
 }
 
 // scalars: [0] = S = sum W C, [1] = Q = sum_j lambda2/J (1 - y_j/q), [2] = q
@@ -607,7 +652,7 @@ struct AdjWork {
     // matrix-core path (d <= 64): per-split partials and the column factors
     bool mfma;
     int rb, nd, n_tiles, n_split, split_cols;
-    double *Vp, *rp, *swp, *bcol, *Yp;
+    double *Vp, *rp, *swp, *bcol, *Yp, *cpart, *cseg;
 };
 
 // matrix-core path: rows per wave, splits and columns per split of an I x J x d
@@ -632,13 +677,15 @@ static size_t adj_carve(Arena &ar, int64_t I, int64_t J, int64_t d, AdjWork *w) 
     w->nparts = w->mfma ? w->n_split * w->n_tiles : (int)((I + kAdjRows - 1) / kAdjRows);
     w->partials = ar.take<double>(w->nparts);
     w->scal = ar.take<double>(8);
-    w->Vp = w->rp = w->swp = w->bcol = w->Yp = nullptr;
+    w->Vp = w->rp = w->swp = w->bcol = w->Yp = w->cpart = w->cseg = nullptr;
     if (w->mfma) {
         w->Vp = ar.take<double>((size_t)w->n_split * I * d);
         w->rp = ar.take<double>((size_t)w->n_split * I);
         w->swp = ar.take<double>((size_t)w->n_split * I);
         w->bcol = ar.take<double>(J);
         w->Yp = ar.take<double>((size_t)J * 16 * w->nd);
+        w->cpart = ar.take<double>((size_t)w->n_tiles * J);
+        w->cseg = ar.take<double>((size_t)((w->n_tiles + kAdjColSeg - 1) / kAdjColSeg) * J);
     }
     return ar.off;
 }
@@ -684,7 +731,8 @@ extern "C" int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int
         adjoint_ypad_kernel<<<(unsigned)((J * 16 * w.nd + 255) / 256), 256, 0, s>>>(Y, (int)J, (int)d, 16 * w.nd, w.Yp);
         WOTK_LAUNCH_CHECK();
         AdjMfmaArgs a{C, ldc, (int)I, (int)J, (int)d, f, g, w.bcol, w.Yp, inv_eps, inv_ij, c_lim, w.split_cols, w.n_tiles,
-                      ldc % 4 == 0 && reinterpret_cast<uintptr_t>(C) % 16 == 0, w.Vp, w.rp, w.swp, w.partials};
+                      ldc % 4 == 0 && reinterpret_cast<uintptr_t>(C) % 16 == 0, w.Vp, w.rp, w.swp, w.partials,
+                      w.cpart};
         switch (w.nd) {
             case 1: launch_adj_mfma<1>(a, w.n_split, s); break;
             case 2: launch_adj_mfma<2>(a, w.n_split, s); break;
@@ -696,6 +744,13 @@ extern "C" int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int
         adjoint_split_sum_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(w.Vp, w.rp, w.swp, w.n_split, (int)I,
                                                                             (int)d, w.V, w.r, w.sw);
         WOTK_LAUNCH_CHECK();
+        // the plan's column sums from the same pass (C is read once)
+        const int n_seg = (w.n_tiles + kAdjColSeg - 1) / kAdjColSeg;
+        adjoint_colseg_kernel<<<dim3((unsigned)((J + 63) / 64), (unsigned)n_seg), 256, 0, s>>>(w.cpart, w.n_tiles,
+                                                                                            (int)J, w.cseg);
+        WOTK_LAUNCH_CHECK();
+        adjoint_colfinal_kernel<<<(unsigned)((J + 255) / 256), 256, 0, s>>>(w.cseg, n_seg, (int)J, w.c);
+        WOTK_LAUNCH_CHECK();
     } else {
         switch ((d + 31) / 32) {
             case 3: adjoint_rows_kernel<3><<<w.nparts, kAdjThreads, 0, s>>>(C, ldc, (int)I, (int)J, f, g, inv_eps,
@@ -706,9 +761,9 @@ extern "C" int wotk_ot_adjoint(const float *C, int64_t ldc, const float *CT, int
                                                                              w.partials); break;
         }
         WOTK_LAUNCH_CHECK();
+        const int rc = launch_plan_sums(CT, ldct, (int)J, (int)I, g, f, epsilon, w.c, s);
+        if (rc) return rc;
     }
-    int rc = launch_plan_sums(CT, ldct, (int)J, (int)I, g, f, epsilon, w.c, s);
-    if (rc) return rc;
     adjoint_scalars_kernel<<<1, 1024, 0, s>>>(w.partials, w.nparts, w.c, (int)I, (int)J, G, lambda2, w.scal);
     WOTK_LAUNCH_CHECK();
     const int64_t n = std::max<int64_t>(I * d, I);
