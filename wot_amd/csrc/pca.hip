@@ -102,6 +102,34 @@ __global__ __launch_bounds__(256) void center_stream_kernel(const T *__restrict_
             }
         }
     };
+    // row sums over the wave, 8 rows at a time: each lane keeps its partial of
+    // every row of the group; a butterfly halves the rows a lane holds at
+    // xor 32, 16 and 8 (exchanging the other half), then sums over xor 4, 2, 1
+    // -- 10 exchanges for 8 rows; row r of the group ends in lane 8 r (fixed order)
+    double rp[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) rp[i] = 0.0;
+    auto flush = [&](int g0) {  // rows g0 .. g0 + 7 of the wave (wave-uniform call)
+        const bool h1 = lane & 32, h2 = lane & 16, h3 = lane & 8;
+        double v4[4], v2[2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double keep = h1 ? rp[4 + i] : rp[i], send = h1 ? rp[i] : rp[4 + i];
+            v4[i] = keep + __shfl_xor(send, 32);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const double keep = h2 ? v4[2 + i] : v4[i], send = h2 ? v4[i] : v4[2 + i];
+            v2[i] = keep + __shfl_xor(send, 16);
+        }
+        double x = (h3 ? v2[1] : v2[0]) + __shfl_xor(h3 ? v2[0] : v2[1], 8);
+#pragma unroll
+        for (int off = 4; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        const int r = lane >> 3;
+        if ((lane & 7) == 0 && g0 + r < rows) rowpart[(int64_t)cb * n + r0 + g0 + r] = x;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) rp[i] = 0.0;
+    };
     auto sum_batch = [&](const V (&bt)[B][NV], int rb) {
 #pragma unroll
         for (int i = 0; i < B; ++i) {
@@ -114,14 +142,16 @@ __global__ __launch_bounds__(256) void center_stream_kernel(const T *__restrict_
                     cs[v][e] += (double)bt[i][v][e];
                     rsum += (double)bt[i][v][e];
                 }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) rsum += __shfl_xor(rsum, off);
-            if (lane == 0) rowpart[(int64_t)cb * n + r0 + rb + i] = rsum;
+            rp[(rb + i) & 7] = rsum;
         }
+        if (((rb + B) & 7) == 0 || rb + B >= rows) flush((rb + B - 1) & ~7);
     };
     V b0[B][NV], b1[B][NV];
     load_batch(b0, 0);
-    for (int rb = 0; rb < rows; rb += 2 * B) {
+    // fully unrolled over the wave's rows, so every row's slot in rp is a constant
+#pragma unroll
+    for (int rb = 0; rb < kStreamRows; rb += 2 * B) {
+        if (rb >= rows) break;  // wave-uniform
         if (rb + B < rows) load_batch(b1, rb + B);
         sum_batch(b0, rb);
         if (rb + B >= rows) break;
